@@ -1,0 +1,55 @@
+"""ctypes binding of the C ABI in include/oslmi.h (liboslmi.so).
+
+The HIP library is the only compute path: if it has not been built, or no
+gfx950 GPU is present, calls raise instead of falling back to anything.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "liboslmi.so")
+
+_lib = None
+
+
+class OSLMIError(RuntimeError):
+    pass
+
+
+def _declare(L):
+    P, I, I64, C = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_char_p
+    sigs = {
+        "oslmi_version": ([], C),
+        "oslmi_last_error": ([], C),
+        "oslmi_init": ([], I),
+        "oslmi_noise": ([C, I, I, I, P, P, I64, P], I),
+        "oslmi_hash": ([I, P, P, I64, P], I),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def lib():
+    """Load liboslmi.so (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSLMIError(
+                "liboslmi.so is not built (%s); run python -m openshadinglanguage_amd.build"
+                % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise OSLMIError("oslmi error %d: %s" % (rc, lib().oslmi_last_error().decode()))
+    return rc
+
+
+def exported_symbols():
+    return [n for n in dir(lib()) if n.startswith("oslmi_")]
