@@ -1,0 +1,40 @@
+// CPU oracle: renderer-side closure evaluation (testrender).
+// TEST INFRASTRUCTURE ONLY (see oracle_core.h).
+#pragma once
+#include "oracle_core.h"
+
+namespace oref {
+
+enum LobeKind { kLobeDiffuse = 0, kLobeGGXRefl = 1 };
+
+struct Lobe {
+    int kind;
+    V3 weight;
+    V3 N;
+    float alpha;  // microfacet xalpha = yalpha
+    float eta;
+};
+
+struct BsdfSample {
+    V3 weight;
+    float pdf;
+};
+
+// Flattened closure list + CompositeBSDF::prepare / eval
+// (src/testrender/shading.h:319-372, MIS::update_eval sampling.h:173-206)
+struct Composite {
+    int n = 0;
+    Lobe lobes[8];
+    float pdfs[8];
+    void add(const Lobe& l)
+    {
+        if (n < 8)
+            lobes[n++] = l;
+    }
+    void prepare(V3 wo);
+    BsdfSample eval(V3 wo, V3 wi) const;
+};
+
+float fresnel_dielectric(float cosi, float eta);  // src/testrender/optics.h:13-31
+
+}  // namespace oref

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "noise.h"
+#include "omath.h"
 
 using namespace oref;
 
@@ -183,6 +184,20 @@ void oref_smoothstep(const float* e0, const float* e1, const float* x,
 {
     for (int64_t i = 0; i < n; ++i)
         out[i] = smoothstep_f(e0[i], e1[i], x[i]);
+}
+
+// The oracle's fixed transcendentals: fn 0 exp, 1 sin, 2 cos, 3 cbrt.
+void oref_math(int fn, const float* in, float* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        float s, c;
+        switch (fn) {
+        case 0: out[i] = ofast_expf(in[i]); break;
+        case 1: ofast_sincosf(in[i], &s, &c); out[i] = s; break;
+        case 2: ofast_sincosf(in[i], &s, &c); out[i] = c; break;
+        default: out[i] = ofast_cbrtf(in[i]); break;
+        }
+    }
 }
 
 }  // extern "C"

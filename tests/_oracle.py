@@ -63,3 +63,77 @@ def hash_op(inp):
     out = np.zeros(inp.shape[1], dtype=np.int32)
     assert lib().oref_hash_op(inp.shape[0], _p(inp), _p(out), inp.shape[1]) == 0
     return out
+
+
+def _setup_more():
+    L = lib()
+    P = ctypes.c_void_p
+    L.oref_gabor.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, ctypes.c_float,
+                             ctypes.c_float, P, P, ctypes.c_int64]
+    L.oref_gabor.restype = ctypes.c_int
+    L.oref_blackbody.argtypes = [P, P, ctypes.c_int64]
+    L.oref_colorsys.argtypes = [ctypes.c_char_p, P]
+    L.oref_shade_c1.argtypes = [P, P, ctypes.c_int64, ctypes.c_int]
+    L.oref_shade_c2.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int64, ctypes.c_int]
+    return L
+
+
+def gabor(inp, anisotropic=0, do_filter=1, direction=(1.0, 0.0, 0.0), bandwidth=1.0,
+          impulses=16.0):
+    """inp: [3*dim, n] Dual2 planes. Returns [3, n] (val, dx, dy)."""
+    L = _setup_more()
+    inp = np.ascontiguousarray(inp, dtype=np.float32)
+    dim = inp.shape[0] // 3
+    out = np.zeros((3, inp.shape[1]), np.float32)
+    d = np.asarray(direction, np.float32)
+    assert L.oref_gabor(dim, anisotropic, do_filter, _p(d), bandwidth, impulses, _p(inp), _p(out),
+                        inp.shape[1]) == 0
+    return out
+
+
+def blackbody(T):
+    L = _setup_more()
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    out = np.zeros((3, T.size), np.float32)
+    L.oref_blackbody(_p(T), _p(out), T.size)
+    return out
+
+
+def colorsys(name="Rec709"):
+    L = _setup_more()
+    out = np.zeros(973, np.float32)
+    L.oref_colorsys(name.encode(), _p(out))
+    return {"bb_table": out[:951].reshape(317, 3), "xyz2rgb": out[951:960].reshape(3, 3),
+            "rgb2xyz": out[960:969].reshape(3, 3), "lum": out[969:972],
+            "bb_entries": int(out[972])}
+
+
+def shade_c1(P, nthreads=1):
+    L = _setup_more()
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    out = np.zeros_like(P)
+    L.oref_shade_c1(_p(P), _p(out), P.shape[1], nthreads)
+    return out
+
+
+def shade_c2(P, N, wo, wi, dPdx, dPdy, nthreads=1, debug=False):
+    L = _setup_more()
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (P, N, wo, wi)]
+    n = arrs[0].shape[1]
+    dx = np.ascontiguousarray(dPdx, np.float32)
+    dy = np.ascontiguousarray(dPdy, np.float32)
+    out = np.zeros((4, n), np.float32)
+    dbg = np.zeros((20, n), np.float32) if debug else None
+    L.oref_shade_c2(*[_p(a) for a in arrs], _p(dx), _p(dy), _p(out),
+                    _p(dbg) if debug else None, n, nthreads)
+    return (out, dbg) if debug else out
+
+
+def omath(fn, x):
+    """The oracle's fixed transcendentals: 'exp', 'sin', 'cos', 'cbrt'."""
+    L = lib()
+    L.oref_math.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros_like(x)
+    L.oref_math(["exp", "sin", "cos", "cbrt"].index(fn), _p(x), _p(out), x.size)
+    return out

@@ -196,3 +196,30 @@ def test_reference_images(test, noisename, offset, scale, maxfrac):
     assert np.mean(diff) < 0.05
     # measured: perlin differs by 1 LSB on 2 of 262144 pixels, cellnoise on none
     assert diff.max() <= 1 and np.mean(diff > 0) < 1e-4
+
+
+def test_gabor_reference_image():
+    """testsuite/noise-gabor (testnoise.osl with noisename gabor, default
+    NoiseParams: isotropic, filtered, bandwidth 1, impulses 16). The float
+    rows of the 1D/2D/3D columns are compared; derivatives of ru/rv/rz follow
+    from testshade's dudx = dvdy = 1/(res-1) through the shader's algebra."""
+    f = np.float32
+    res, inscale = 512, 20.0
+    U, V, ru, rv, rz, tt = _testnoise_coords(res)
+    dud = f(1.0) / f(res - 1)
+    theta = f(10.0) * f(np.pi / 180.0)
+    c, sn = np.cos(theta, dtype=f), np.sin(theta, dtype=f)
+    dx = [(c * dud) * f(inscale), (sn * dud) * f(inscale), (f(inscale) * dud) / f(np.sqrt(2))]
+    dy = [(sn * dud) * f(inscale), (-(c * dud)) * f(inscale), (f(inscale) * dud) / f(np.sqrt(2))]
+    ref = np.load(os.path.join(HERE, "golden", "noise_images.npz"))["noise_gabor"].astype(np.int32)
+    cols = {1: U < 0.24, 2: (U > 0.26) & (U < 0.49), 3: (U > 0.51) & (U < 0.74)}
+    for dim in (1, 2, 3):
+        m = cols[dim] & (V < 0.49)
+        k = int(m.sum())
+        vals = [ru[m], rv[m], rz[m]][:dim]
+        inp = np.stack(vals + [np.full(k, d, f) for d in dx[:dim]] + [np.full(k, d, f) for d in dy[:dim]])
+        r = O.gabor(inp, anisotropic=0)
+        got = _to_u8(f(0.5) + f(0.5) * r[0])
+        diff = np.abs(got - ref[..., 0][m])
+        # measured: exact except one pixel (1 LSB) in the filtered 2D column
+        assert diff.max() <= 1 and np.mean(diff > 0) < 1e-4, (dim, diff.max())

@@ -1,0 +1,83 @@
+"""Pin the oracle's transcendental approximations, color system and closure
+lobes to what mathematics fixes (libm / float64 closed forms)."""
+import numpy as np
+import pytest
+
+import _oracle as O
+
+
+def _ulps(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float64).astype(np.float32)
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    ai = np.where(ai < 0, -(ai & 0x7fffffff), ai)
+    bi = np.where(bi < 0, -(bi & 0x7fffffff), bi)
+    return np.abs(ai - bi)
+
+
+def setup_module(module):
+    O.lib()
+
+
+def test_exp_accuracy():
+    x = np.linspace(-87, 88, 200001).astype(np.float32)
+    got = O.omath("exp", x)
+    ref = np.exp(x.astype(np.float64))
+    assert _ulps(got, ref).max() <= 2
+    assert O.omath("exp", np.array([0.0], np.float32))[0] == 1.0
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_sincos_accuracy(fn):
+    x = np.linspace(-200, 200, 400001).astype(np.float32)
+    got = O.omath(fn, x).astype(np.float64)
+    ref = getattr(np, fn)(x.astype(np.float64))
+    # absolute error bound relative to the unit range (near zeros ulps blow up)
+    assert np.abs(got - ref).max() < 2.5e-7
+
+
+def test_cbrt_accuracy():
+    x = np.concatenate([np.linspace(1e-3, 6000, 100001), [8.0, 27.0, 5600.0]]).astype(np.float32)
+    got = O.omath("cbrt", x)
+    assert _ulps(got, np.cbrt(x.astype(np.float64))).max() <= 2
+
+
+def _planck_xyz(T):
+    """float64 Planck integration with the reference's 5 nm CIE sampling."""
+    from importlib import import_module  # noqa
+    cie = np.array(_CIE, np.float64).reshape(81, 3)
+    lam = (380.0 + 5.0 * np.arange(81)) * 1e-9
+    c1, c2 = 3.74183e-16, 1.4388e-2
+    spec = c1 / lam ** 5 / np.expm1(c2 / (lam * T))
+    return (spec[:, None] * 5e-9 * cie).sum(0)
+
+
+_CIE = None
+
+
+def test_blackbody_matches_planck():
+    global _CIE
+    import re
+    import os
+    # the CIE table the oracle uses is data from opcolor_impl.h:55-90; read it
+    # back out of the oracle source to avoid a second copy
+    src = open(os.path.join(os.path.dirname(O.LIB), "..", "color.cpp")).read()
+    body = src[src.index("kCIE[81 * 3] = {"):]
+    body = body[:body.index("};")]
+    _CIE = [float(v) for v in re.findall(r"\d+\.\d+", body)]
+    assert len(_CIE) == 243
+    cs = O.colorsys("Rec709")
+    M = cs["xyz2rgb"].astype(np.float64)
+    Ts = np.array([1000.0, 1500.0, 2500.0, 4000.0, 5500.0, 6500.0, 8000.0, 9999.0], np.float32)
+    got = O.blackbody(Ts).T.astype(np.float64)
+    for T, g in zip(Ts, got):
+        ref = np.maximum(_planck_xyz(float(T)) @ M, 0.0)
+        # the table is accurate to ~0.1% (opcolor_impl.h:144-145); allow 0.5%
+        # of the brightest channel (dim channels interpolate less accurately)
+        assert np.all(np.abs(g - ref) <= 5e-3 * ref.max()), (T, g, ref)
+    # Rec709 luminance weights are the well-known (0.2126, 0.7152, 0.0722)
+    assert np.allclose(cs["lum"], [0.2126, 0.7152, 0.0722], atol=2e-4)
+    # XYZ->RGB of the D65 white point maps to (1,1,1)
+    w = np.array([0.3127 / 0.3291, 1.0, (1 - 0.3127 - 0.3291) / 0.3291])
+    assert np.allclose(w @ M, 1.0, atol=1e-4)
