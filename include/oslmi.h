@@ -52,6 +52,24 @@ int oslmi_noise(const char* name, int outdim, int indim, int derivs, const float
 /* hash() shadeop over 1..4 float planes (src/liboslexec/wide/wide_ophash.cpp:28-80). */
 int oslmi_hash(int indim, const float* d_in, int32_t* d_out, int64_t n, void* stream);
 
+/* ---- Prebuilt group lowerings -----------------------------------------
+ * The two benchmark groups compiled ahead of time (OSL sources in
+ * openshadinglanguage_amd/shaders/). Each replaces one
+ * BatchedExecutor<W>::execute over the whole point set
+ * (src/include/OSL/oslexec.h:1000-1004) plus, for layered3, the renderer's
+ * closure evaluation (src/testrender/shading.h:364-372).
+ *
+ * fbm_cell: d_P 3 planes in; d_Cout 3 planes out. */
+int oslmi_shade_fbm_cell(int64_t n, const float* d_P, float* d_Cout, void* stream);
+
+/* layered3: d_P, d_N, d_wo, d_wi 3 planes each; dPdx/dPdy host float[3]
+ * (uniform screen derivatives of P); d_out 4 planes (R, G, B, pdf);
+ * d_aov optional (NULL or 20 planes: fbm v/dx/dy, vn v xyz/dx xyz/dy xyz,
+ * gabor v/dx/dy, Cd xyz, alpha, 0). */
+int oslmi_shade_layered3(int64_t n, const float* d_P, const float* d_N, const float* d_wo,
+                         const float* d_wi, const float* dPdx, const float* dPdy, float* d_out,
+                         float* d_aov, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

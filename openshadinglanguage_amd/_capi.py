@@ -24,6 +24,8 @@ def _declare(L):
         "oslmi_init": ([], I),
         "oslmi_noise": ([C, I, I, I, P, P, I64, P], I),
         "oslmi_hash": ([I, P, P, I64, P], I),
+        "oslmi_shade_fbm_cell": ([I64, P, P, P], I),
+        "oslmi_shade_layered3": ([I64, P, P, P, P, P, P, P, P, P], I),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

@@ -1,0 +1,187 @@
+// CDNA4 device library: closure lobes and their MIS combination.
+//
+// OSL builds a per-point closure tree (ADD/MUL nodes over ClosureComponents,
+// src/include/OSL/oslclosure.h:41-145); the renderer flattens it to at most
+// eight weighted lobes and evaluates them with multiple-importance weights
+// (testrender: CompositeBSDF, src/testrender/shading.h:319-437). On the GPU
+// the flattened form is the only form: a fixed-capacity lobe list held in
+// registers, built directly by the emitted shader code.
+//
+//   Diffuse<0>                 src/testrender/shading.cpp:301-312
+//   Microfacet<GGXDist,0>      src/testrender/shading.cpp:572-615, :713-757
+//   GGXDist                    src/testrender/shading.cpp:497-506
+//   TangentFrame::from_normal  src/testrender/sampling.h:17-42
+//   fresnel_dielectric         src/testrender/optics.h:13-31
+//   CompositeBSDF prepare/eval src/testrender/shading.h:323-372
+//   MIS::update_eval           src/testrender/sampling.h:173-206
+#pragma once
+#include "oslmi_types.h"
+
+namespace oslmi {
+
+// Closure ids as registered by testrender (shading.cpp:185-296 order).
+enum ClosureId { kClosureDiffuse = 1, kClosureMicrofacetGGX = 2 };
+
+struct Lobe {
+    int id;
+    V3 weight;
+    V3 N;
+    float alpha;
+    float eta;
+};
+
+template<int MAXL> struct LobeList {
+    int n;
+    Lobe l[MAXL];
+    float pdf[MAXL];
+    OSLMI_DEV LobeList() : n(0) {}
+    OSLMI_DEV void add(const Lobe& x)
+    {
+        if (n < MAXL)
+            l[n++] = x;
+    }
+};
+
+struct BsdfEval {
+    V3 weight;
+    float pdf;
+};
+
+OSLMI_DEV float fresnel_dielectric(float cosi, float eta)
+{
+    if (eta == 0)
+        return 1;
+    if (cosi < 0.0f)
+        eta = 1.0f / eta;
+    float c = fabsf(cosi);
+    float g = eta * eta - 1 + c * c;
+    if (g > 0) {
+        g       = sqrtf(g);
+        float A = (g - c) / (g + c);
+        float B = (c * (g + c) - 1) / (c * (g - c) + 1);
+        return 0.5f * A * A * (1 + B * B);
+    }
+    return 1.0f;
+}
+
+struct Frame {
+    V3 u, v, w;
+};
+// from_normal_and_tangent(n, 0) falls back to from_normal (Duff et al.)
+OSLMI_DEV Frame frame_from_normal(V3 n)
+{
+    const float sign = copysignf(1.0f, n.z);
+    const float a    = -1 / (sign + n.z);
+    const float b    = n.x * n.y * a;
+    Frame f;
+    f.u = V3(1 + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f.v = V3(b, sign + n.y * n.y * a, -n.y);
+    f.w = n;
+    return f;
+}
+OSLMI_DEV V3 tolocal(const Frame& f, V3 a) { return V3(dot(a, f.u), dot(a, f.v), dot(a, f.w)); }
+
+OSLMI_DEV float ggx_lambda(V3 w, float xa, float ya)
+{
+    float cosTheta2  = w.z * w.z;
+    float cosPhi2st2 = (w.x * xa) * (w.x * xa);
+    float sinPhi2st2 = (w.y * ya) * (w.y * ya);
+    float a2         = cosTheta2 / (cosPhi2st2 + sinPhi2st2);
+    return 0.5f * (-1.0f + sqrtf(1.0f + 1.0f / a2));
+}
+
+OSLMI_DEV float ggx_D(V3 Hr, float xa, float ya)
+{
+    float cosThetaM = Hr.z;
+    if (cosThetaM > 0) {
+        float cosPhi2st2 = (Hr.x / xa) * (Hr.x / xa);
+        float sinPhi2st2 = (Hr.y / ya) * (Hr.y / ya);
+        float cosThetaM2 = cosThetaM * cosThetaM;
+        float cosThetaM4 = cosThetaM2 * cosThetaM2;
+        float tanThetaM2 = (cosPhi2st2 + sinPhi2st2) / cosThetaM2;
+        float F          = 1 / (3.14159274101257324219f * (1 + tanThetaM2) * (1 + tanThetaM2));
+        return F / (xa * ya * cosThetaM4);
+    }
+    return 0;
+}
+
+OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
+{
+    BsdfEval r;
+    if (l.id == kClosureDiffuse) {
+        r.weight = V3(1.0f);
+        r.pdf    = fmaxf(dot(l.N, wi), 0.0f) * 0.31830987334251403809f;  // float(M_1_PI)
+        return r;
+    }
+    Frame tf = frame_from_normal(l.N);
+    V3 wo_l  = tolocal(tf, wo);
+    V3 wi_l  = tolocal(tf, wi);
+    if (wo_l.z > 0 && wi_l.z > 0) {
+        V3 h     = wi_l + wo_l;
+        float hl = sqrtf(dot(h, h));
+        V3 m     = hl != 0.0f ? V3(h.x / hl, h.y / hl, h.z / hl) : h;
+        float D  = ggx_D(m, l.alpha, l.alpha);
+        float Lo = ggx_lambda(wo_l, l.alpha, l.alpha);
+        float Li = ggx_lambda(wi_l, l.alpha, l.alpha);
+        float G2 = 1 / (Lo + Li + 1);
+        float G1 = 1 / (Lo + 1);
+        float Fr = fresnel_dielectric(dot(m, wo_l), l.eta);
+        r.pdf    = (G1 * D * 0.25f) / wo_l.z;
+        r.weight = V3((G2 / G1) * Fr);
+        return r;
+    }
+    r.weight = V3(0.0f);
+    r.pdf    = 0.0f;
+    return r;
+}
+
+OSLMI_DEV V3 lobe_albedo(const Lobe& l, V3 wo)
+{
+    if (l.id == kClosureDiffuse)
+        return V3(1.0f);
+    return V3(fresnel_dielectric(dot(l.N, wo), l.eta));
+}
+
+template<int MAXL> OSLMI_DEV void composite_prepare(LobeList<MAXL>& L, V3 wo)
+{
+    const V3 pw(1.0f);
+    float total = 0;
+    for (int i = 0; i < L.n; i++) {
+        L.pdf[i] = dot(L.l[i].weight, pw * lobe_albedo(L.l[i], wo)) / (pw.x + pw.y + pw.z);
+        total += L.pdf[i];
+    }
+    if (total > 0 || total > 1)
+        for (int i = 0; i < L.n; i++)
+            L.pdf[i] /= total;
+}
+
+OSLMI_DEV void mis_update(V3* w, float* pdf, V3 ow, float opdf, float b)
+{
+    if (b > 1.17549435e-38f) {
+        opdf *= b;
+        ow = ow * (1 / b);
+        float mis;
+        if (*pdf < opdf)
+            mis = 1 / (1 + *pdf / opdf);
+        else if (opdf < *pdf)
+            mis = 1 - 1 / (1 + opdf / *pdf);
+        else
+            mis = 0.5f;
+        *w = *w * (1 - mis) + ow * mis;
+        *pdf += opdf;
+    }
+}
+
+template<int MAXL> OSLMI_DEV BsdfEval composite_eval(const LobeList<MAXL>& L, V3 wo, V3 wi)
+{
+    BsdfEval s;
+    s.weight = V3(0.0f);
+    s.pdf    = 0.0f;
+    for (int i = 0; i < L.n; i++) {
+        BsdfEval b = lobe_eval(L.l[i], wo, wi);
+        mis_update(&s.weight, &s.pdf, b.weight * L.l[i].weight, b.pdf, L.pdf[i]);
+    }
+    return s;
+}
+
+}  // namespace oslmi

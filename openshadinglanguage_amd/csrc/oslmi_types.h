@@ -8,6 +8,12 @@
 // reference's evaluation order.
 #pragma once
 
+#if !defined(__HIPCC_RTC__)
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#endif
+
 #ifndef OSLMI_DEV
 #define OSLMI_DEV __device__ __forceinline__
 #endif

@@ -82,7 +82,9 @@ float bb_table_unmap(float T)
     return ic * ic;
 }
 
-float bb_spectrum(float T, float lambda_nm)
+// table build (host, once): libm expm1f; per point (T >= 12000 K): the
+// oracle's fixed expm1.
+float bb_spectrum(float T, float lambda_nm, bool per_point)
 {
     float wlm       = lambda_nm * 1e-9f;
     const float c1  = 3.74183e-16f;
@@ -91,16 +93,17 @@ float bb_spectrum(float T, float lambda_nm)
     float wlm4      = wlm2 * wlm2;
     float wlm5      = wlm4 * wlm;
     float inv_wlm5  = 1.0f / wlm5;
-    return (c1 * inv_wlm5) / expm1f(c2 / (wlm * T));
+    float x         = c2 / (wlm * T);
+    return (c1 * inv_wlm5) / (per_point ? ofast_expm1f(x) : expm1f(x));
 }
 
-V3 spectrum_to_XYZ(float T)
+V3 spectrum_to_XYZ(float T, bool per_point = false)
 {
     float X = 0, Y = 0, Z = 0;
     const float dlambda = 5.0f * 1e-9;
     for (int i = 0; i < 81; ++i) {
         float lambda = 380.0f + 5.0f * (float)i;
-        float Me     = bb_spectrum(T, lambda) * dlambda;
+        float Me     = bb_spectrum(T, lambda, per_point) * dlambda;
         X += Me * kCIE[3 * i + 0];
         Y += Me * kCIE[3 * i + 1];
         Z += Me * kCIE[3 * i + 2];
@@ -223,7 +226,7 @@ V3 blackbody_rgb(const ColorSys& cs, float T)
         V3 rgb4 = rgb2 * rgb2;
         return rgb4 * rgb;
     }
-    V3 rgb = mul_row(spectrum_to_XYZ(T), cs.xyz2rgb);
+    V3 rgb = mul_row(spectrum_to_XYZ(T, true), cs.xyz2rgb);
     clamp_zero(rgb);
     return rgb;
 }

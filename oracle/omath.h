@@ -92,6 +92,24 @@ inline void ofast_sincosf(float x, float* s, float* c)
     *c = cv;
 }
 
+// e^x - 1: Taylor series to x^9 for |x| < 0.5, exp(x) - 1 otherwise.
+inline float ofast_expm1f(float x)
+{
+    if (fabsf(x) < 0.5f) {
+        float p = 1.0f / 362880.0f;
+        p       = fmaf(p, x, 1.0f / 40320.0f);
+        p       = fmaf(p, x, 1.0f / 5040.0f);
+        p       = fmaf(p, x, 1.0f / 720.0f);
+        p       = fmaf(p, x, 1.0f / 120.0f);
+        p       = fmaf(p, x, 1.0f / 24.0f);
+        p       = fmaf(p, x, 1.0f / 6.0f);
+        p       = fmaf(p, x, 0.5f);
+        p       = fmaf(p, x, 1.0f);
+        return p * x;
+    }
+    return ofast_expf(x) - 1.0f;
+}
+
 // Cube root: bit-level initial guess, then three Newton steps
 // y <- y - (y^3 - x) / (3 y^2) for x > 0 (sign restored for x < 0).
 inline float ofast_cbrtf(float x)
