@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: Mpts/s shaded on one MI355X vs the batched CPU path.
+
+Default workload (N=1): the 3-layer "layered3" group over 1024x1024 points,
+derivatives on, closure evaluated at one (wo, wi) pair per point (one step =
+one batched execution of the group over all points). --workload fbm_cell
+runs the single-layer 256x256 group instead.
+
+Prints one JSON line (see the README of the benchmark contract): metric,
+value (Mpts/s, whole job), ms_per_step, roofline (VALU-bound kernel, live
+HIP-event timing) and cpu_baseline (the CPU oracle timed on the host cores
+on the same workload).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X (gfx950) 32-bit VALU peak: 256 CUs x 128 lanes/clk x 2.4 GHz =
+# 78.6 T lane-ops/s (the published 157.3 TFLOP/s FP32 vector figure counts an
+# FMA as two flops). Measured on the box: k_layered3 issues 1.24 VALU
+# wave-instructions per CU-clock (SQ_INSTS_VALU / GRBM_GUI_ACTIVE), which
+# only a 2-per-clock issue rate allows.
+VALU_PEAK_OPS = 256 * 128 * 2.4e9
+HBM_PEAK_BPS = 8.0e12
+
+# Algorithmic 32-bit lane-ops per point (oracle/count/count_ops.py ->
+# openshadinglanguage_amd/roofline_ops.json) and algorithmic HBM bytes per
+# point (SoA planes read + written).
+WORKLOADS = {
+    "layered3": {"res": 1024, "bytes_per_point": (12 * 4) + 16},
+    "fbm_cell": {"res": 256, "bytes_per_point": 12 + 12},
+}
+
+# HBM traffic per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (profiles/*_pmc.txt), in bytes; None when not measured.
+# FETCH_SIZE reads half the bytes of a plain 256 MiB copy on this GPU (x2
+# scale), WRITE_SIZE reads them exactly (x1); profiles/layered3_pmc.txt.
+MEASURED_TRAFFIC = {"layered3": 2 * 24816.98 * 1024 + 16384.0 * 1024, "fbm_cell": None}
+
+
+def _ops_per_point(name):
+    path = os.path.join(ROOT, "openshadinglanguage_amd", "roofline_ops.json")
+    return json.load(open(path))[name]["ops_per_point"]
+
+
+def cpu_baseline(name, inputs, nthreads, passes):
+    """The CPU oracle (reference-algorithm restatement) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    O.lib()
+    if name == "layered3":
+        n = inputs["P"].shape[1]
+        fn = lambda: O.shade_c2(inputs["P"], inputs["N"], inputs["wo"], inputs["wi"],  # noqa
+                                inputs["dPdx"], inputs["dPdy"], nthreads=nthreads)
+    else:
+        n = inputs["P"].shape[1]
+        fn = lambda: O.shade_c1(inputs["P"], nthreads=nthreads)  # noqa
+    fn()  # warm (thread creation, page faults)
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        fn()
+    dt = time.perf_counter() - t0
+    return {"value": round(n * passes / dt / 1e6, 3), "unit": "Mpts/s", "cores": nthreads,
+            "kind": "port",
+            "sample": "%d pass(es) over the full %d-point workload, %d threads, %.1f s wall"
+                      % (passes, n, nthreads, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="layered3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 256 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the shading path has no "
+              "cross-GPU exchange; --gpus %d is not supported)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if args.copy_calibration:
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 4, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": nbytes, "bytes_written": nbytes,
+                          "launches": args.steps}))
+        return
+
+    from openshadinglanguage_amd import check, lib
+    from openshadinglanguage_amd import workloads as W
+
+    check(lib().oslmi_init())
+    dev = torch.device("cuda:0")
+    name = args.workload
+    res = WORKLOADS[name]["res"]
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    if name == "layered3":
+        inp = W.layered3_inputs(res, seed=2)
+        n = inp["P"].shape[1]
+        t = {k: torch.from_numpy(inp[k]).to(dev) for k in ("P", "N", "wo", "wi")}
+        out = torch.empty((4, n), device=dev)
+        dx = inp["dPdx"].ctypes.data_as(ctypes.c_void_p)
+        dy = inp["dPdy"].ctypes.data_as(ctypes.c_void_p)
+
+        def step():
+            check(lib().oslmi_shade_layered3(n, p(t["P"]), p(t["N"]), p(t["wo"]), p(t["wi"]),
+                                             dx, dy, p(out), None, sp))
+        config = {"workload": "layered3: 3-layer group (simplex fBm 6 oct + vector perlin + "
+                              "anisotropic 2D gabor -> mix/smoothstep/blackbody/transformc -> "
+                              "diffuse*Cd + ggx microfacet, 1 closure eval/pt), derivs on",
+                  "points": n, "grid": "%dx%d" % (res, res), "precision": "fp32"}
+    else:
+        inp = W.fbm_cell_inputs(res, seed=1)
+        n = inp["P"].shape[1]
+        Pt = torch.from_numpy(inp["P"]).to(dev)
+        out = torch.empty_like(Pt)
+
+        def step():
+            check(lib().oslmi_shade_fbm_cell(n, p(Pt), p(out), sp))
+        config = {"workload": "fbm_cell: 4-octave perlin fBm + 0.25 cellnoise(3P) -> "
+                              "smoothstep/mix, no derivs",
+                  "points": n, "grid": "%dx%d" % (res, res), "precision": "fp32"}
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    value = n / (ms_per_step / 1e3) / 1e6
+
+    ops = _ops_per_point(name)
+    achieved = ops * n / (kernel_ms / 1e3)
+    roof = {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+            "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
+            "frac": round(achieved / VALU_PEAK_OPS, 4),
+            "traffic": MEASURED_TRAFFIC[name],
+            "ops_per_point": round(ops, 1),
+            "kernel_ms": round(kernel_ms, 4),
+            "hbm_algorithmic_GBps": round(WORKLOADS[name]["bytes_per_point"] * n
+                                          / (kernel_ms / 1e3) / 1e9, 2),
+            "hbm_peak_GBps": HBM_PEAK_BPS / 1e9}
+
+    line = {"metric": "Mpts/s shaded (GPU vs host batched CPU)", "value": round(value, 2),
+            "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)",
+            "config": config, "roofline": roof}
+    if not args.no_cpu:
+        nthreads = min(16, os.cpu_count() or 1)
+        cpu_inp = inp if name == "fbm_cell" else inp
+        line["cpu_baseline"] = cpu_baseline(name, cpu_inp, nthreads, args.cpu_passes)
+        line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,64 @@
+#!/usr/bin/env python3
+"""Algorithmic 32-bit lane-operations per shading point of the benchmark
+groups, counted by the op-counting build of the oracle
+(oracle/build/liboslref_count.so, see counted.h), on seeded samples of the
+benchmark workloads. Also checks that the counting build computes the same
+values as the oracle. Writes openshadinglanguage_amd/roofline_ops.json.
+
+    make -C oracle count && python oracle/count/count_ops.py
+"""
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import _oracle as O  # noqa: E402
+from openshadinglanguage_amd import workloads as W  # noqa: E402
+
+
+def main():
+    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "liboslref_count.so"))
+    P = ctypes.c_void_p
+    L.ocount_get.restype = ctypes.c_ulonglong
+    L.oref_shade_c1.argtypes = [P, P, ctypes.c_int64, ctypes.c_int]
+    L.oref_shade_c2.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int64, ctypes.c_int]
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    res = {}
+
+    w = W.fbm_cell_inputs(256, 1)
+    out = np.zeros_like(w["P"])
+    L.ocount_reset()
+    L.oref_shade_c1(p(w["P"]), p(out), w["P"].shape[1], 1)
+    n = w["P"].shape[1]
+    res["fbm_cell"] = {"ops_per_point": L.ocount_get() / n, "points_counted": n}
+    assert np.array_equal(out, O.shade_c1(w["P"])), "counting build differs from oracle"
+
+    full = W.layered3_inputs(1024, 2)
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(full["P"].shape[1], 32768, replace=False))
+    sub = {k: np.ascontiguousarray(v[:, idx]) if v.ndim == 2 else v for k, v in full.items()}
+    n = idx.size
+    out = np.zeros((4, n), np.float32)
+    L.ocount_reset()
+    L.oref_shade_c2(p(sub["P"]), p(sub["N"]), p(sub["wo"]), p(sub["wi"]), p(sub["dPdx"]),
+                    p(sub["dPdy"]), p(out), None, n, 1)
+    res["layered3"] = {"ops_per_point": L.ocount_get() / n, "points_counted": int(n),
+                       "sample": "32768 random points of the 1024x1024 workload (seed 0)"}
+    ref = O.shade_c2(sub["P"], sub["N"], sub["wo"], sub["wi"], sub["dPdx"], sub["dPdy"])
+    assert np.array_equal(out, ref), "counting build differs from oracle"
+    res["_doc"] = ("algorithmic 32-bit lane-ops per point from the op-counting oracle "
+                   "(oracle/count/count_ops.py)")
+    dst = os.path.join(ROOT, "openshadinglanguage_amd", "roofline_ops.json")
+    json.dump(res, open(dst, "w"), indent=1)
+    print(json.dumps(res, indent=1))
+
+
+if __name__ == "__main__":
+    main()

@@ -186,12 +186,51 @@ struct FastRng {
     }
 };
 
+// filter_gabor_kernel_2d split in two: the part that depends only on the
+// filter, the bandwidth and the impulse's 2D frequency (FilterConsts), and
+// the per-impulse weight  w_f = ((c_F * w) * K) * E  (filter_weight). With a
+// fixed anisotropy direction the first part is the same for every impulse
+// of a point and is evaluated once per point; the values are unchanged.
+struct FilterConsts {
+    float c_F, K, E, a_f, omf_x, omf_y;
+};
+
+FilterConsts filter_consts(const M22& filter, float a, float ox, float oy)
+{
+    FilterConsts f;
+    const M22& Sigma_f = filter;
+    M22 Sigma_G        = (a * a / kTwoPi) * M22();
+    f.c_F              = 1.0f / (kTwoPi * sqrtf(determinant(Sigma_f)));
+    M22 Sigma_F        = float(1.0 / (4.0 * M_PI * M_PI)) * inverse(Sigma_f);
+    M22 SGF            = Sigma_G + Sigma_F;
+    M22 SGFi           = inverse(SGF);
+    // gabor_mul_m22_v2(M, v) = (v.x*M00 + v.y*M10, v.x*M01 + v.y*M11)
+    float mx = ox * SGFi.m[0][0] + oy * SGFi.m[1][0];
+    float my = ox * SGFi.m[0][1] + oy * SGFi.m[1][1];
+    f.E      = ofast_expf(-0.5f * (mx * ox + my * oy));
+    f.K      = 1.0f / (kTwoPi * sqrtf(determinant(SGF)));
+    M22 Sigma_G_i   = inverse(Sigma_G);
+    M22 Sigma_GF    = inverse(inverse(Sigma_F) + Sigma_G_i);
+    M22 Sigma_GF_Gi = Sigma_GF * Sigma_G_i;
+    f.omf_x         = ox * Sigma_GF_Gi.m[0][0] + oy * Sigma_GF_Gi.m[1][0];
+    f.omf_y         = ox * Sigma_GF_Gi.m[0][1] + oy * Sigma_GF_Gi.m[1][1];
+    f.a_f           = sqrtf((float)(2.0 * M_PI * (double)sqrtf(determinant(Sigma_GF))));
+    return f;
+}
+
+D2<float> filter_weight(const FilterConsts& f, const D2<float>& w)
+{
+    return (f.c_F * w) * f.K * f.E;
+}
+
 struct Params {
     M22 filter;
     M33 local;
     V3 N;
     V3 omega;
     int do_filter;
+    V3 omega_t;       // anisotropic: local * omega
+    FilterConsts fc;  // anisotropic: filter_consts(filter, a, omega_t.xy)
 };
 
 // gabor_kernel with a 3D (dual) position
@@ -241,28 +280,6 @@ void setup_filter(const D2<V3>& P, Params& gp)
     gp.do_filter = do_filter;
 }
 
-void filter_kernel_2d(const M22& filter, const D2<float>& w, float a, float ox, float oy,
-                      D2<float>& w_f, float& a_f, float& omf_x, float& omf_y)
-{
-    const M22& Sigma_f = filter;
-    M22 Sigma_G        = (a * a / kTwoPi) * M22();
-    float c_F          = 1.0f / (kTwoPi * sqrtf(determinant(Sigma_f)));
-    M22 Sigma_F        = float(1.0 / (4.0 * M_PI * M_PI)) * inverse(Sigma_f);
-    M22 SGF            = Sigma_G + Sigma_F;
-    M22 SGFi           = inverse(SGF);
-    // gabor_mul_m22_v2(M, v) = (v.x*M00 + v.y*M10, v.x*M01 + v.y*M11)
-    float mx = ox * SGFi.m[0][0] + oy * SGFi.m[1][0];
-    float my = ox * SGFi.m[0][1] + oy * SGFi.m[1][1];
-    float e  = ofast_expf(-0.5f * (mx * ox + my * oy));
-    w_f      = (c_F * w) * (1.0f / (kTwoPi * sqrtf(determinant(SGF)))) * e;
-    M22 Sigma_G_i   = inverse(Sigma_G);
-    M22 Sigma_GF    = inverse(inverse(Sigma_F) + Sigma_G_i);
-    M22 Sigma_GF_Gi = Sigma_GF * Sigma_G_i;
-    omf_x           = ox * Sigma_GF_Gi.m[0][0] + oy * Sigma_GF_Gi.m[1][0];
-    omf_y           = ox * Sigma_GF_Gi.m[0][1] + oy * Sigma_GF_Gi.m[1][1];
-    a_f             = sqrtf((float)(2.0 * M_PI * (double)sqrtf(determinant(Sigma_GF))));
-}
-
 D2<float> cell(const GaborUniform& gup, const GaborOpts& opt, const Params& gp, V3 c_i,
                const D2<V3>& x_c_i)
 {
@@ -296,16 +313,23 @@ D2<float> cell(const GaborUniform& gup, const GaborOpts& opt, const Params& gp, 
             }
             phi = kTwoPi * rng();
             if (opt.do_filter && gp.do_filter == 1) {
-                V3 omega_t    = mult(gp.local, omega);
+                V3 omega_t;
+                FilterConsts fc;
+                if (opt.anisotropic == 1) {
+                    omega_t = gp.omega_t;
+                    fc      = gp.fc;
+                } else {
+                    omega_t = mult(gp.local, omega);
+                    fc      = filter_consts(gp.filter, gup.a, omega_t.x, omega_t.y);
+                }
                 D2<float> d_i = -ddot(gp.N, xk);
                 // slice_gabor_kernel_3d
                 D2<float> w_s   = 1.0f * dexp((float(-kPi) * (gup.a * gup.a)) * (d_i * d_i));
                 D2<float> phi_s = phi - kTwoPi * d_i * omega_t.z;
-                D2<float> w_f;
-                float a_f, ofx, ofy;
-                filter_kernel_2d(gp.filter, w_s, gup.a, omega_t.x, omega_t.y, w_f, a_f, ofx, ofy);
+                D2<float> w_f   = filter_weight(fc, w_s);
                 D2<V3> xkit  = mult(gp.local, xk);
-                D2<float> gk = kernel2(w_f, ofx, ofy, phi_s, a_f, comp(xkit, 0), comp(xkit, 1));
+                D2<float> gk = kernel2(w_f, fc.omf_x, fc.omf_y, phi_s, fc.a_f, comp(xkit, 0),
+                                       comp(xkit, 1));
                 if (!std::isfinite(gk.v))
                     gk = kernel3(D2<float>(1.0f), omega, D2<float>(phi), gup.a, xk);
                 sum = sum + gk;
@@ -349,15 +373,20 @@ GaborUniform gabor_uniform(const GaborOpts& opt)
 D2<float> gabor(int dim, const D2<float>* p, const GaborOpts& opt)
 {
     GaborUniform gup = gabor_uniform(opt);
-    D2<V3> P(V3(p[0].v, dim > 1 ? p[1].v : 0.0f, dim > 2 ? p[2].v : 0.0f),
-             V3(p[0].dx, dim > 1 ? p[1].dx : 0.0f, dim > 2 ? p[2].dx : 0.0f),
-             V3(p[0].dy, dim > 1 ? p[1].dy : 0.0f, dim > 2 ? p[2].dy : 0.0f));
+    const float zero = 0.0f;
+    D2<V3> P(V3(p[0].v, dim > 1 ? p[1].v : zero, dim > 2 ? p[2].v : zero),
+             V3(p[0].dx, dim > 1 ? p[1].dx : zero, dim > 2 ? p[2].dx : zero),
+             V3(p[0].dy, dim > 1 ? p[1].dy : zero, dim > 2 ? p[2].dy : zero));
     Params gp;
     gp.omega     = opt.direction;
     gp.do_filter = 0;
     gp.N         = V3(0.0f);
     if (opt.do_filter)
         setup_filter(P, gp);
+    if (gp.do_filter == 1 && opt.anisotropic == 1) {
+        gp.omega_t = mult(gp.local, gp.omega);
+        gp.fc      = filter_consts(gp.filter, gup.a, gp.omega_t.x, gp.omega_t.y);
+    }
     // gabor_evaluate + gabor_grid
     D2<V3> xg = P * gup.radius_inv;
     V3 fl(floorf(xg.v.x), floorf(xg.v.y), floorf(xg.v.z));

@@ -8,8 +8,9 @@ uint32_t inthash_n(const uint32_t* k, int n)
 {
     // reference_inthash<N>, src/include/OSL/oslnoise.h:192-215
     uint32_t a, b, c;
-    uint32_t len = (uint32_t)n;
-    a = b = c = 0xdeadbeefu + (len << 2) + 13u;
+    const unsigned seed = 0xdeadbeefu + ((unsigned)n << 2) + 13u;  // a constant per N
+    uint32_t len        = (uint32_t)n;
+    a = b = c = seed;
     while (len > 3) {
         a += k[0];
         b += k[1];
@@ -90,15 +91,28 @@ template<class T> struct Ops;
 template<> struct Ops<float> {
     static float one() { return 1.0f; }
     static float c(float x) { return x; }
-    static float neg_if(float v, bool b) { return b ? -v : v; }
-    static float sel(bool b, float t, float f) { return b ? t : f; }
+    static float neg_if(float v, bool b)
+    {
+        OREF_COUNT(1);
+        return b ? -v : v;
+    }
+    static float sel(bool b, float t, float f)
+    {
+        OREF_COUNT(1);
+        return b ? t : f;
+    }
 };
 template<> struct Ops<D2<float>> {
     static D2<float> one() { return D2<float>(1.0f); }
     static D2<float> c(float x) { return D2<float>(x); }
-    static D2<float> neg_if(const D2<float>& v, bool b) { return b ? -v : v; }
+    static D2<float> neg_if(const D2<float>& v, bool b)
+    {
+        OREF_COUNT(3);
+        return b ? -v : v;
+    }
     static D2<float> sel(bool b, const D2<float>& t, const D2<float>& f)
     {
+        OREF_COUNT(3);
         return b ? t : f;
     }
 };
@@ -146,6 +160,7 @@ static inline T trilerp(const T& v0, const T& v1, const T& v2, const T& v3,
 
 template<class T> static inline T grad(int hash, const T& x)
 {
+    OREF_COUNT(5);
     int h   = hash & 15;
     float g = (float)(1 + (h & 7));
     if (h & 8)
@@ -154,6 +169,7 @@ template<class T> static inline T grad(int hash, const T& x)
 }
 template<class T> static inline T grad(int hash, const T& x, const T& y)
 {
+    OREF_COUNT(4);
     int h = hash & 7;
     T u   = Ops<T>::sel(h < 4, x, y);
     T v   = 2.0f * Ops<T>::sel(h < 4, y, x);
@@ -161,6 +177,7 @@ template<class T> static inline T grad(int hash, const T& x, const T& y)
 }
 template<class T> static inline T grad(int hash, const T& x, const T& y, const T& z)
 {
+    OREF_COUNT(8);
     int h = hash & 15;
     T u   = Ops<T>::sel(h < 8, x, y);
     T v   = Ops<T>::sel(h < 4, y, Ops<T>::sel((h == 12) || (h == 14), x, z));
@@ -169,6 +186,7 @@ template<class T> static inline T grad(int hash, const T& x, const T& y, const T
 template<class T>
 static inline T grad(int hash, const T& x, const T& y, const T& z, const T& w)
 {
+    OREF_COUNT(7);
     int h = hash & 31;
     T u   = Ops<T>::sel(h < 24, x, y);
     T v   = Ops<T>::sel(h < 16, y, z);
@@ -501,7 +519,9 @@ static float simplex3(float x, float y, float z, int seed, float* dnoise)
     float x3 = x0 - 1.0f + 3.0f * G3;
     float y3 = y0 - 1.0f + 3.0f * G3;
     float z3 = z0 - 1.0f + 3.0f * G3;
+    OREF_COUNT(15);  // simplex corner ordering logic (bg*/i1..k2)
     auto g3  = [&](int a, int b, int c, float* g) {
+        OREF_COUNT(9);  // corner index adds, table decode (3 components)
         uint32_t h = scramble((uint32_t)a, (uint32_t)b,
                               scramble((uint32_t)c, (uint32_t)seed));
         g[0]       = kGrad3[h & 15][0];

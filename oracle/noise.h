@@ -8,7 +8,17 @@ namespace oref {
 // --- Jenkins lookup3 mix/final -------------------------------------------
 // src/include/OSL/oslnoise.h:154-180 restates OIIO::bjhash::bjmix/bjfinal
 // (Bob Jenkins, lookup3.c, "mix" and "final").
+#ifdef OREF_COUNTING
+// a rotate is one instruction (v_alignbit_b32 on CDNA)
+inline uint32_t rotl32(uint32_t x, int k)
+{
+    OREF_COUNT(1);
+    unsigned v = x.v;
+    return uint32_t((v << k) | (v >> (32 - k)));
+}
+#else
 inline uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+#endif
 
 inline void bjmix(uint32_t& a, uint32_t& b, uint32_t& c)
 {

@@ -13,6 +13,12 @@
 #include <cstdint>
 #include <cstring>
 
+// Op-count marks for integer selection logic (see count/counted.h); no-ops
+// in the normal build.
+#ifndef OREF_COUNT
+#define OREF_COUNT(k) ((void)0)
+#endif
+
 namespace oref {
 
 struct V3 {
