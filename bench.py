@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="layered3", choices=sorted(WORKLOADS))
+    ap.add_argument("--path", default="jit", choices=["jit", "prebuilt"],
+                    help="jit: group built from OSO and lowered by the emitter (default); "
+                         "prebuilt: the ahead-of-time lowering")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -122,9 +125,16 @@ def main():
         dx = inp["dPdx"].ctypes.data_as(ctypes.c_void_p)
         dy = inp["dPdy"].ctypes.data_as(ctypes.c_void_p)
 
-        def step():
-            check(lib().oslmi_shade_layered3(n, p(t["P"]), p(t["N"]), p(t["wo"]), p(t["wi"]),
-                                             dx, dy, p(out), None, sp))
+        if args.path == "jit":
+            from openshadinglanguage_amd import shadingsys as S
+            ss, grp = S.build_layered3()
+            ss.jit_group(grp)
+            gl = {"P": t["P"], "N": t["N"], "dPdx": tuple(inp["dPdx"]), "dPdy": tuple(inp["dPdy"])}
+            step = ss.bind(grp, n, gl, wo=t["wo"], wi=t["wi"], eval_out=out, stream=stream)
+        else:
+            def step():
+                check(lib().oslmi_shade_layered3(n, p(t["P"]), p(t["N"]), p(t["wo"]),
+                                                 p(t["wi"]), dx, dy, p(out), None, sp))
         config = {"workload": "layered3: 3-layer group (simplex fBm 6 oct + vector perlin + "
                               "anisotropic 2D gabor -> mix/smoothstep/blackbody/transformc -> "
                               "diffuse*Cd + ggx microfacet, 1 closure eval/pt), derivs on",
@@ -135,12 +145,21 @@ def main():
         Pt = torch.from_numpy(inp["P"]).to(dev)
         out = torch.empty_like(Pt)
 
-        def step():
-            check(lib().oslmi_shade_fbm_cell(n, p(Pt), p(out), sp))
+        if args.path == "jit":
+            from openshadinglanguage_amd import shadingsys as S
+            ss, grp = S.build_fbm_cell()
+            ss.jit_group(grp)
+
+            step = ss.bind(grp, n, {"P": Pt}, outputs=out, stream=stream)
+        else:
+            def step():
+                check(lib().oslmi_shade_fbm_cell(n, p(Pt), p(out), sp))
         config = {"workload": "fbm_cell: 4-octave perlin fBm + 0.25 cellnoise(3P) -> "
                               "smoothstep/mix, no derivs",
                   "points": n, "grid": "%dx%d" % (res, res), "precision": "fp32"}
 
+    config["path"] = ("OSO group -> emitter -> hiprtc kernel" if args.path == "jit"
+                      else "prebuilt lowering")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

@@ -52,6 +52,72 @@ int oslmi_noise(const char* name, int outdim, int indim, int derivs, const float
 /* hash() shadeop over 1..4 float planes (src/liboslexec/wide/wide_ophash.cpp:28-80). */
 int oslmi_hash(int indim, const float* d_in, int32_t* d_out, int64_t n, void* stream);
 
+/* ---- Shading system, shader groups, batched execution -----------------
+ * The drop-in path: shader masters are loaded from OSO text, assembled into
+ * groups of layers with parameter values and connections, lowered by the
+ * library's emitter to one fused gfx950 kernel (hiprtc) and executed over
+ * any number of points. */
+typedef struct oslmi_shadingsys oslmi_shadingsys;
+typedef struct oslmi_group oslmi_group;
+
+/* Shader globals of a batch (BatchedShaderGlobals<W>, src/include/OSL/
+ * batched_shaderglobals.h:21-140) as SoA planes over n points. Index order:
+ * P, dPdx, dPdy, N, Ng, I, u, v, dudx, dudy, dvdx, dvdy, time, dPdu, dPdv, Ps.
+ * planes[k] is a device pointer (3 planes for triples, 1 for floats) or NULL,
+ * in which case uniform[k] holds the value shared by all points (uniform
+ * shader globals, batched_shaderglobals.h:21-75). */
+#define OSLMI_NUM_GLOBALS 16
+typedef struct oslmi_batched_globals {
+    int64_t n;
+    const float* planes[OSLMI_NUM_GLOBALS];
+    float uniform[OSLMI_NUM_GLOBALS][3];
+} oslmi_batched_globals;
+
+/* ShadingSystem construction (src/include/OSL/oslexec.h:170). include_dir:
+ * directory of the device library headers (the package's csrc/) handed to
+ * the JIT; cache_dir: code-object cache directory or NULL. */
+oslmi_shadingsys* oslmi_shadingsys_create(const char* include_dir, const char* cache_dir);
+void oslmi_shadingsys_destroy(oslmi_shadingsys* ss);
+/* ShadingSystem::attribute (oslexec.h:177-324): "colorspace", "jit_cache". */
+int oslmi_shadingsys_attribute(oslmi_shadingsys* ss, const char* name, const char* value);
+/* LoadMemoryCompiledShader: register a shader master from OSO text. */
+int oslmi_load_memory_compiled_shader(oslmi_shadingsys* ss, const char* shadername,
+                                      const char* oso_text);
+/* ShaderGroupBegin / Parameter / Shader / ConnectShaders / ShaderGroupEnd
+ * (oslexec.h:634-742). Parameter values apply to the next oslmi_shader
+ * call. type: "int", "float", "color", "point", "vector", "normal",
+ * "matrix", "string" (value: const char* const*). */
+oslmi_group* oslmi_shader_group_begin(oslmi_shadingsys* ss, const char* groupname);
+int oslmi_parameter(oslmi_shadingsys* ss, oslmi_group* g, const char* name, const char* type,
+                    const void* value, int count);
+int oslmi_shader(oslmi_shadingsys* ss, oslmi_group* g, const char* usage, const char* shadername,
+                 const char* layername);
+int oslmi_connect_shaders(oslmi_shadingsys* ss, oslmi_group* g, const char* srclayer,
+                          const char* srcparam, const char* dstlayer, const char* dstparam);
+int oslmi_shader_group_end(oslmi_shadingsys* ss, oslmi_group* g);
+void oslmi_group_release(oslmi_group* g);
+/* Renderer outputs ("renderer_outputs"): symbols written to the output
+ * planes in order ("layer.symbol" or "symbol" = last layer that has it);
+ * a triple takes 3 planes, an int/float 1. */
+int oslmi_group_outputs(oslmi_shadingsys* ss, oslmi_group* g, int n, const char* const* names);
+/* jit_group (oslexec.h:993-998): emit + compile + load. */
+int oslmi_jit_group(oslmi_shadingsys* ss, oslmi_group* g);
+/* Emit and compile to a gfx950 code object without loading it (no device
+ * needed); fills the code-object cache so a later jit_group only loads. */
+int oslmi_compile_group(oslmi_shadingsys* ss, oslmi_group* g);
+/* Emit only (no device needed); copies the HIP source into buf and returns
+ * its length (debug_output of the reference). */
+int oslmi_group_emit(oslmi_shadingsys* ss, oslmi_group* g, char* buf, int64_t bufsize);
+int oslmi_group_output_planes(oslmi_shadingsys* ss, oslmi_group* g);
+double oslmi_group_jit_seconds(oslmi_group* g);
+/* BatchedExecutor<W>::execute over bg->n points (oslexec.h:1000-1004).
+ * d_outputs: output_planes planes of n floats. If d_eval is non-NULL the
+ * surface closure Ci is evaluated at (d_wo, d_wi) (3 planes each) like the
+ * renderer's CompositeBSDF::eval and written as 4 planes (R, G, B, pdf). */
+int oslmi_execute_batch(oslmi_shadingsys* ss, oslmi_group* g, const oslmi_batched_globals* bg,
+                        float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
+                        void* stream);
+
 /* ---- Prebuilt group lowerings -----------------------------------------
  * The two benchmark groups compiled ahead of time (OSL sources in
  * openshadinglanguage_amd/shaders/). Each replaces one

@@ -26,6 +26,23 @@ def _declare(L):
         "oslmi_hash": ([I, P, P, I64, P], I),
         "oslmi_shade_fbm_cell": ([I64, P, P, P], I),
         "oslmi_shade_layered3": ([I64, P, P, P, P, P, P, P, P, P], I),
+        "oslmi_shadingsys_create": ([C, C], P),
+        "oslmi_shadingsys_destroy": ([P], None),
+        "oslmi_shadingsys_attribute": ([P, C, C], I),
+        "oslmi_load_memory_compiled_shader": ([P, C, C], I),
+        "oslmi_shader_group_begin": ([P, C], P),
+        "oslmi_parameter": ([P, P, C, C, P, I], I),
+        "oslmi_shader": ([P, P, C, C, C], I),
+        "oslmi_connect_shaders": ([P, P, C, C, C, C], I),
+        "oslmi_shader_group_end": ([P, P], I),
+        "oslmi_group_release": ([P], None),
+        "oslmi_group_outputs": ([P, P, I, P], I),
+        "oslmi_jit_group": ([P, P], I),
+        "oslmi_compile_group": ([P, P], I),
+        "oslmi_group_emit": ([P, P, P, I64], I),
+        "oslmi_group_output_planes": ([P, P], I),
+        "oslmi_group_jit_seconds": ([P], ctypes.c_double),
+        "oslmi_execute_batch": ([P, P, P, P, P, P, P, P], I),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

@@ -1,0 +1,82 @@
+// Shading system, shader groups and the group emitter (host side).
+//
+// A ShadingSys owns shader masters (parsed OSO) and groups. A group is an
+// ordered list of layers (shader instances with parameter values) plus
+// connections between layer outputs and inputs (ShaderGroupBegin / Parameter
+// / Shader / ConnectShaders / ShaderGroupEnd, src/include/OSL/oslexec.h:
+// 634-742). jit() lowers the whole group to one HIP kernel (emit.cpp) and
+// compiles it with hiprtc for gfx950; execute() runs it over n points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "oslmi_groupargs.h"
+#include "oslmi_oso.h"
+
+namespace oslmi {
+
+struct ParamValue {
+    OType type;
+    std::vector<float> f;
+    std::vector<int> i;
+    std::vector<std::string> s;
+};
+
+struct Layer {
+    std::string name;
+    const ShaderMaster* master = nullptr;
+    std::map<std::string, ParamValue> inst;  // instance parameter values
+};
+
+struct Connection {
+    int src_layer, src_sym, dst_layer, dst_sym;
+};
+
+// Where an output lands in the caller's plane array.
+struct OutputSlot {
+    std::string name;
+    int layer, sym;
+    int planes;  // 1 (int/float) or 3 (triple)
+    int first_plane;
+};
+
+struct Group {
+    std::string name;
+    bool ended = false;
+    std::vector<Layer> layers;
+    std::vector<Connection> conns;
+    std::vector<std::string> output_names;
+    // compiled state
+    bool jitted = false;
+    std::string source;
+    std::vector<OutputSlot> outputs;
+    int output_planes        = 0;
+    bool surface_closure     = false;  // last layer writes Ci
+    bool uses_colorsys       = false;
+    std::vector<unsigned> globals_used;  // GlobalIndex values read
+    hipModule_t module       = nullptr;
+    hipFunction_t kernel     = nullptr;
+    double jit_seconds       = 0.0;
+    std::mutex mu;
+    int find_layer(const std::string& n) const;
+};
+
+struct ShadingSys {
+    std::map<std::string, std::unique_ptr<ShaderMaster>> masters;
+    std::vector<std::unique_ptr<Group>> groups;
+    std::string colorspace = "Rec709";
+    std::string cache_dir;   // code-object cache ("" = none)
+    std::string include_dir; // device library headers for hiprtc
+    std::mutex mu;
+};
+
+// Emits the HIP source of the group's fused kernel; fills g.outputs,
+// g.output_planes, g.surface_closure, g.uses_colorsys, g.globals_used.
+bool emit_group(const ShadingSys& ss, Group& g, std::string& src, std::string& err);
+
+}  // namespace oslmi

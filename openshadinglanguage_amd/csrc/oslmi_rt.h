@@ -1,0 +1,247 @@
+// Device runtime for emitted group kernels.
+//
+// The group emitter lowers each OSO op to a call of one of the functions
+// below (or of the shadeop library: oslmi_noise.h, oslmi_gabor.h,
+// oslmi_color.h, oslmi_closure.h). Symbols are registers of type int,
+// float, V3 (point/vector/normal/color), their Dual2 forms Df/Dv when the
+// derivative analysis says they carry derivatives, M44 (matrix) or Closure
+// (a flattened lobe list). Mixed-type operands are promoted by the emitter
+// before the call (float -> triple broadcast, value -> Dual2 with zero
+// derivatives), matching the reference's batched code generation
+// (src/liboslexec/batched_llvm_gen.cpp).
+#pragma once
+#include "oslmi_closure.h"
+#include "oslmi_color.h"
+#include "oslmi_gabor.h"
+#include "oslmi_math.h"
+#include "oslmi_noise.h"
+
+namespace oslmi {
+namespace rt {
+
+// ---------------------------------------------------------- promotion
+OSLMI_DEV float val(float x) { return x; }
+OSLMI_DEV float val(const Df& x) { return x.v; }
+OSLMI_DEV V3 val(V3 x) { return x; }
+OSLMI_DEV V3 val(const Dv& x) { return x.v; }
+OSLMI_DEV Df dual(float x) { return Df(x); }
+OSLMI_DEV Df dual(const Df& x) { return x; }
+OSLMI_DEV Dv dual(V3 x) { return Dv(x); }
+OSLMI_DEV Dv dual(const Dv& x) { return x; }
+OSLMI_DEV V3 tri(float x) { return V3(x, x, x); }
+OSLMI_DEV V3 tri(V3 x) { return x; }
+OSLMI_DEV Dv tri(const Df& x) { return Dv(V3(x.v), V3(x.dx), V3(x.dy)); }
+OSLMI_DEV Dv tri(const Dv& x) { return x; }
+OSLMI_DEV float tofloat(int x) { return (float)x; }
+OSLMI_DEV float tofloat(float x) { return x; }
+
+// ---------------------------------------------------------- arithmetic
+OSLMI_DEV float add(float a, float b) { return a + b; }
+OSLMI_DEV V3 add(V3 a, V3 b) { return a + b; }
+OSLMI_DEV Df add(const Df& a, const Df& b) { return a + b; }
+OSLMI_DEV Dv add(const Dv& a, const Dv& b) { return a + b; }
+OSLMI_DEV int add(int a, int b) { return a + b; }
+OSLMI_DEV float sub(float a, float b) { return a - b; }
+OSLMI_DEV V3 sub(V3 a, V3 b) { return a - b; }
+OSLMI_DEV Df sub(const Df& a, const Df& b) { return a - b; }
+OSLMI_DEV Dv sub(const Dv& a, const Dv& b) { return a - b; }
+OSLMI_DEV int sub(int a, int b) { return a - b; }
+OSLMI_DEV float mul(float a, float b) { return a * b; }
+OSLMI_DEV V3 mul(V3 a, V3 b) { return a * b; }
+OSLMI_DEV int mul(int a, int b) { return a * b; }
+OSLMI_DEV Df mul(const Df& a, const Df& b) { return a * b; }
+// componentwise product rule for triples with derivatives
+OSLMI_DEV Dv mul(const Dv& a, const Dv& b)
+{
+    return Dv(a.v * b.v, a.v * b.dx + a.dx * b.v, a.v * b.dy + a.dy * b.v);
+}
+OSLMI_DEV float div(float a, float b) { return a / b; }
+OSLMI_DEV V3 div(V3 a, V3 b) { return V3(a.x / b.x, a.y / b.y, a.z / b.z); }
+OSLMI_DEV int div(int a, int b) { return b == 0 ? 0 : a / b; }
+OSLMI_DEV Df div(const Df& a, const Df& b) { return a / b; }
+OSLMI_DEV Dv div(const Dv& a, const Dv& b)
+{
+    Df x = comp(a, 0) / comp(b, 0), y = comp(a, 1) / comp(b, 1), z = comp(a, 2) / comp(b, 2);
+    return make_dv(x, y, z);
+}
+OSLMI_DEV float neg(float a) { return -a; }
+OSLMI_DEV V3 neg(V3 a) { return -a; }
+OSLMI_DEV Df neg(const Df& a) { return -a; }
+OSLMI_DEV Dv neg(const Dv& a) { return -a; }
+OSLMI_DEV int neg(int a) { return -a; }
+
+// OSL mod: a - b*floor(a/b) (safe: 0 when b == 0)
+OSLMI_DEV float fmod_osl(float a, float b) { return b != 0.0f ? a - b * floorf(a / b) : 0.0f; }
+
+// ---------------------------------------------------------- comparisons
+template<class A, class B> OSLMI_DEV int lt(A a, B b) { return val(a) < val(b); }
+template<class A, class B> OSLMI_DEV int gt(A a, B b) { return val(a) > val(b); }
+template<class A, class B> OSLMI_DEV int le(A a, B b) { return val(a) <= val(b); }
+template<class A, class B> OSLMI_DEV int ge(A a, B b) { return val(a) >= val(b); }
+OSLMI_DEV int lt(int a, int b) { return a < b; }
+OSLMI_DEV int gt(int a, int b) { return a > b; }
+OSLMI_DEV int le(int a, int b) { return a <= b; }
+OSLMI_DEV int ge(int a, int b) { return a >= b; }
+OSLMI_DEV int eq(float a, float b) { return a == b; }
+OSLMI_DEV int eq(int a, int b) { return a == b; }
+OSLMI_DEV int neq(float a, float b) { return a != b; }
+OSLMI_DEV int neq(int a, int b) { return a != b; }
+
+// ---------------------------------------------------------- mix / smoothstep
+// llvm_gen_mix (batched_llvm_gen.cpp:1868-1960)
+OSLMI_DEV float mix(float a, float b, float x) { return a * (1.0f - x) + b * x; }
+OSLMI_DEV V3 mix(V3 a, V3 b, V3 x)
+{
+    return V3(mix(a.x, b.x, x.x), mix(a.y, b.y, x.y), mix(a.z, b.z, x.z));
+}
+OSLMI_DEV Df mix(const Df& a, const Df& b, const Df& x)
+{
+    float omx = 1.0f - x.v;
+    float rx  = ((a.dx * omx - a.v * x.dx) + b.v * x.dx) + b.dx * x.v;
+    float ry  = ((a.dy * omx - a.v * x.dy) + b.v * x.dy) + b.dy * x.v;
+    return Df(a.v * omx + b.v * x.v, rx, ry);
+}
+OSLMI_DEV Dv mix(const Dv& a, const Dv& b, const Dv& x)
+{
+    return make_dv(mix(comp(a, 0), comp(b, 0), comp(x, 0)), mix(comp(a, 1), comp(b, 1), comp(x, 1)),
+                   mix(comp(a, 2), comp(b, 2), comp(x, 2)));
+}
+// smoothstep on floats / duals: oslmi_noise.h; triples componentwise
+OSLMI_DEV V3 smoothstep(V3 e0, V3 e1, V3 x)
+{
+    return V3(oslmi::smoothstep(e0.x, e1.x, x.x), oslmi::smoothstep(e0.y, e1.y, x.y),
+              oslmi::smoothstep(e0.z, e1.z, x.z));
+}
+
+// min/max/clamp: max(min(x, hi), lo) (stdosl.h:140-142); duals select whole
+OSLMI_DEV float fmin_(float a, float b) { return b < a ? b : a; }
+OSLMI_DEV float fmax_(float a, float b) { return a < b ? b : a; }
+OSLMI_DEV float min_(float a, float b) { return fminf(a, b); }
+OSLMI_DEV float max_(float a, float b) { return fmaxf(a, b); }
+OSLMI_DEV int min_(int a, int b) { return a < b ? a : b; }
+OSLMI_DEV int max_(int a, int b) { return a > b ? a : b; }
+OSLMI_DEV V3 min_(V3 a, V3 b) { return V3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+OSLMI_DEV V3 max_(V3 a, V3 b) { return V3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+OSLMI_DEV Df min_(const Df& a, const Df& b) { return b.v < a.v ? b : a; }
+OSLMI_DEV Df max_(const Df& a, const Df& b) { return a.v < b.v ? b : a; }
+template<class T> OSLMI_DEV T clamp(const T& x, const T& lo, const T& hi) { return max_(min_(x, hi), lo); }
+
+// ---------------------------------------------------------- math
+OSLMI_DEV float floor_(float x) { return floorf(x); }
+OSLMI_DEV V3 floor_(V3 x) { return V3(floorf(x.x), floorf(x.y), floorf(x.z)); }
+OSLMI_DEV float abs_(float x) { return fabsf(x); }
+OSLMI_DEV int abs_(int x) { return x < 0 ? -x : x; }
+OSLMI_DEV float sqrt_(float x) { return x > 0.0f ? sqrtf(x) : 0.0f; }
+OSLMI_DEV Df sqrt_(const Df& a)
+{
+    if (a.v > 0.0f) {
+        float f = sqrtf(a.v), df = 0.5f / f;
+        return Df(f, df * a.dx, df * a.dy);
+    }
+    return Df(0.0f);
+}
+OSLMI_DEV float exp_(float x) { return fexp(x); }
+OSLMI_DEV Df exp_(const Df& a) { return dexp(a); }
+OSLMI_DEV float sin_(float x) { float s, c; fsincos(x, &s, &c); return s; }
+OSLMI_DEV float cos_(float x) { float s, c; fsincos(x, &s, &c); return c; }
+OSLMI_DEV Df cos_(const Df& a) { return dcos(a); }
+OSLMI_DEV Df sin_(const Df& a)
+{
+    float s, c;
+    fsincos(a.v, &s, &c);
+    return Df(s, c * a.dx, c * a.dy);
+}
+OSLMI_DEV float dot_(V3 a, V3 b) { return dot(a, b); }
+OSLMI_DEV Df dot_(const Dv& a, const Dv& b) { return ddot(a, b); }
+OSLMI_DEV float length_(V3 a) { return sqrtf(dot(a, a)); }
+OSLMI_DEV V3 normalize_(V3 a) { return vnormalized(a); }
+OSLMI_DEV V3 cross_(V3 a, V3 b) { return cross(a, b); }
+// filterwidth (llvm_ops.cpp:946-956): sqrt(dx^2 + dy^2)
+OSLMI_DEV float filterwidth(const Df& x) { return sqrtf(x.dx * x.dx + x.dy * x.dy); }
+OSLMI_DEV V3 filterwidth(const Dv& x)
+{
+    return V3(sqrtf(x.dx.x * x.dx.x + x.dy.x * x.dy.x), sqrtf(x.dx.y * x.dx.y + x.dy.y * x.dy.y),
+              sqrtf(x.dx.z * x.dx.z + x.dy.z * x.dy.z));
+}
+OSLMI_DEV float Dx(const Df& x) { return x.dx; }
+OSLMI_DEV float Dy(const Df& x) { return x.dy; }
+OSLMI_DEV V3 Dx(const Dv& x) { return x.dx; }
+OSLMI_DEV V3 Dy(const Dv& x) { return x.dy; }
+
+// ---------------------------------------------------------- components
+OSLMI_DEV float compref(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+OSLMI_DEV Df compref(const Dv& a, int i) { return comp(a, i); }
+OSLMI_DEV void compassign(V3& a, int i, float x)
+{
+    if (i == 0) a.x = x; else if (i == 1) a.y = x; else a.z = x;
+}
+OSLMI_DEV void compassign(Dv& a, int i, const Df& x)
+{
+    if (i == 0) { a.v.x = x.v; a.dx.x = x.dx; a.dy.x = x.dy; }
+    else if (i == 1) { a.v.y = x.v; a.dx.y = x.dx; a.dy.y = x.dy; }
+    else { a.v.z = x.v; a.dx.z = x.dx; a.dy.z = x.dy; }
+}
+OSLMI_DEV V3 triple(float a, float b, float c) { return V3(a, b, c); }
+OSLMI_DEV Dv triple(const Df& a, const Df& b, const Df& c) { return make_dv(a, b, c); }
+
+// ---------------------------------------------------------- matrices
+struct M44 {
+    float m[16];
+};
+// transform of a point (w = 1, homogeneous divide), vector, normal by a
+// row-vector convention matrix (Imath::Matrix44 multVecMatrix/multDirMatrix).
+OSLMI_DEV V3 xform_point(const M44& M, V3 p)
+{
+    const float* m = M.m;
+    float a = p.x * m[0] + p.y * m[4] + p.z * m[8] + m[12];
+    float b = p.x * m[1] + p.y * m[5] + p.z * m[9] + m[13];
+    float c = p.x * m[2] + p.y * m[6] + p.z * m[10] + m[14];
+    float w = p.x * m[3] + p.y * m[7] + p.z * m[11] + m[15];
+    return V3(a / w, b / w, c / w);
+}
+OSLMI_DEV V3 xform_vector(const M44& M, V3 p)
+{
+    const float* m = M.m;
+    return V3(p.x * m[0] + p.y * m[4] + p.z * m[8], p.x * m[1] + p.y * m[5] + p.z * m[9],
+              p.x * m[2] + p.y * m[6] + p.z * m[10]);
+}
+
+// ---------------------------------------------------------- closures
+typedef LobeList<8> Closure;
+OSLMI_DEV Closure closure_none() { return Closure(); }
+OSLMI_DEV Closure closure_diffuse(V3 N)
+{
+    Closure c;
+    Lobe l;
+    l.id = kClosureDiffuse; l.weight = V3(1.0f); l.N = N; l.alpha = 0.0f; l.eta = 0.0f;
+    c.add(l);
+    return c;
+}
+OSLMI_DEV Closure closure_microfacet_ggx(V3 N, float alpha, float eta)
+{
+    Closure c;
+    Lobe l;
+    l.id = kClosureMicrofacetGGX; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
+    c.add(l);
+    return c;
+}
+// color * closure (a MUL node): scales every lobe weight
+OSLMI_DEV Closure mul(const Closure& c, V3 w)
+{
+    Closure r = c;
+    for (int i = 0; i < r.n; ++i)
+        r.l[i].weight = r.l[i].weight * w;
+    return r;
+}
+OSLMI_DEV Closure mul(V3 w, const Closure& c) { return mul(c, w); }
+// ADD node: lobes of a then b (depth-first order of process_bsdf_closure)
+OSLMI_DEV Closure add(const Closure& a, const Closure& b)
+{
+    Closure r = a;
+    for (int i = 0; i < b.n; ++i)
+        r.add(b.l[i]);
+    return r;
+}
+
+}  // namespace rt
+}  // namespace oslmi

@@ -1,0 +1,434 @@
+// ShadingSystem / ShaderGroup / BatchedExecutor behind the C ABI.
+//
+//   oslmi_shadingsys_create        ShadingSystem::ShadingSystem (oslexec.h:170)
+//   oslmi_shadingsys_attribute     ShadingSystem::attribute (oslexec.h:177-324)
+//   oslmi_load_memory_compiled_shader  LoadMemoryCompiledShader (oslexec.h)
+//   oslmi_shader_group_begin/end   ShaderGroupBegin / ShaderGroupEnd (:634-742)
+//   oslmi_parameter                Parameter(group, name, TypeDesc, val)
+//   oslmi_shader                   Shader(group, usage, shadername, layername)
+//   oslmi_connect_shaders          ConnectShaders(group, srclayer, srcparam, ...)
+//   oslmi_group_outputs            attribute(group, "renderer_outputs", ...)
+//   oslmi_jit_group                BatchedExecutor<W>::jit_group (:993-998)
+//   oslmi_execute_batch            BatchedExecutor<W>::execute (:1000-1004)
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <sstream>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "oslmi.h"
+#include "oslmi_color.h"
+#include "oslmi_colorsys.h"
+#include "oslmi_group.h"
+#include "oslmi_internal.h"
+
+namespace oslmi {
+
+int device_colorsys(const ColorSysDev** out);  // oslmi_prebuilt.hip
+
+namespace {
+
+bool parse_type(const char* t, OType& out)
+{
+    std::string s(t ? t : "");
+    if (s == "int") out.base = kInt;
+    else if (s == "float") out.base = kFloat;
+    else if (s == "color" || s == "point" || s == "vector" || s == "normal") {
+        out.base        = kTriple;
+        out.triple_kind = s;
+    } else if (s == "string") out.base = kString;
+    else if (s == "matrix") out.base = kMatrix;
+    else return false;
+    return true;
+}
+
+std::string hex64(uint64_t h)
+{
+    char b[17];
+    snprintf(b, sizeof(b), "%016llx", (unsigned long long)h);
+    return b;
+}
+
+uint64_t fnv1a(const std::string& s)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+bool read_file(const std::string& path, std::vector<char>& out)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        return false;
+    out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return !out.empty();
+}
+
+void mkdirs(const std::string& d)
+{
+    std::string cur;
+    for (size_t i = 0; i < d.size(); ++i) {
+        cur += d[i];
+        if (d[i] == '/' || i + 1 == d.size())
+            mkdir(cur.c_str(), 0755);
+    }
+}
+
+const char* kCompileOpts[] = { "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops" };
+
+int compile_group(ShadingSys& ss, Group& g, bool load = true)
+{
+    std::string src, err;
+    if (!emit_group(ss, g, src, err))
+        return set_error(OSLMI_ERR_COMPILE, "emit " + g.name + ": " + err);
+    g.source = src;
+    std::string opts_key;
+    for (const char* o : kCompileOpts)
+        opts_key += std::string(o) + " ";
+    // include the device library headers in the cache key
+    std::string hdrs;
+    for (const char* h : { "oslmi_rt.h", "oslmi_types.h", "oslmi_noise.h", "oslmi_gabor.h",
+                           "oslmi_math.h", "oslmi_color.h", "oslmi_closure.h", "oslmi_groupargs.h" }) {
+        std::vector<char> c;
+        read_file(ss.include_dir + "/" + h, c);
+        hdrs.append(c.begin(), c.end());
+    }
+    std::string key = hex64(fnv1a(src + "|" + opts_key + "|" + hdrs));
+    std::vector<char> code;
+    std::string cache_path = ss.cache_dir.empty() ? "" : ss.cache_dir + "/" + key + ".hsaco";
+    auto t0 = std::chrono::steady_clock::now();
+    if (cache_path.empty() || !read_file(cache_path, code)) {
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, src.c_str(), (g.name + ".hip").c_str(), 0, nullptr, nullptr)
+            != HIPRTC_SUCCESS)
+            return set_error(OSLMI_ERR_COMPILE, "hiprtcCreateProgram failed");
+        std::vector<const char*> opts(std::begin(kCompileOpts), std::end(kCompileOpts));
+        std::string inc = "-I" + ss.include_dir;
+        opts.push_back(inc.c_str());
+        hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+        size_t logsz    = 0;
+        hiprtcGetProgramLogSize(prog, &logsz);
+        std::string log(logsz, '\0');
+        if (logsz)
+            hiprtcGetProgramLog(prog, &log[0]);
+        if (rc != HIPRTC_SUCCESS) {
+            hiprtcDestroyProgram(&prog);
+            return set_error(OSLMI_ERR_COMPILE, "hiprtc compile of group " + g.name + " failed:\n" + log);
+        }
+        size_t sz = 0;
+        hiprtcGetCodeSize(prog, &sz);
+        code.resize(sz);
+        hiprtcGetCode(prog, code.data());
+        hiprtcDestroyProgram(&prog);
+        if (!cache_path.empty()) {
+            mkdirs(ss.cache_dir);
+            std::string tmp = cache_path + ".tmp" + std::to_string((long)getpid());
+            std::ofstream f(tmp, std::ios::binary);
+            f.write(code.data(), (std::streamsize)code.size());
+            f.close();
+            rename(tmp.c_str(), cache_path.c_str());
+        }
+    }
+    g.jit_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!load)
+        return OSLMI_OK;
+    if (g.module) {
+        (void)hipModuleUnload(g.module);
+        g.module = nullptr;
+    }
+    if (int rc = check_hip(hipModuleLoadData(&g.module, code.data()), "hipModuleLoadData"))
+        return rc;
+    if (int rc = check_hip(hipModuleGetFunction(&g.kernel, g.module, "oslmi_group_main"),
+                           "hipModuleGetFunction"))
+        return rc;
+    g.jitted = true;
+    return OSLMI_OK;
+}
+
+// Compiles without a device: emits and runs hiprtc, filling the cache.
+int precompile_group(ShadingSys& ss, Group& g)
+{
+    std::string src, err;
+    if (!emit_group(ss, g, src, err))
+        return set_error(OSLMI_ERR_COMPILE, "emit " + g.name + ": " + err);
+    g.source = src;
+    return OSLMI_OK;
+}
+
+}  // namespace
+}  // namespace oslmi
+
+using namespace oslmi;
+
+struct oslmi_shadingsys {
+    ShadingSys ss;
+};
+struct oslmi_group {
+    Group* g;
+};
+
+extern "C" {
+
+oslmi_shadingsys* oslmi_shadingsys_create(const char* include_dir, const char* cache_dir)
+{
+    auto* s           = new oslmi_shadingsys;
+    s->ss.include_dir = include_dir ? include_dir : "";
+    s->ss.cache_dir   = cache_dir ? cache_dir : "";
+    return s;
+}
+
+void oslmi_shadingsys_destroy(oslmi_shadingsys* s)
+{
+    if (!s)
+        return;
+    for (auto& g : s->ss.groups)
+        if (g->module)
+            (void)hipModuleUnload(g->module);
+    delete s;
+}
+
+int oslmi_shadingsys_attribute(oslmi_shadingsys* s, const char* name, const char* value)
+{
+    if (!s || !name || !value)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_shadingsys_attribute: null argument");
+    std::string n(name);
+    if (n == "colorspace") {
+        if (std::string(value) != "Rec709")
+            return set_error(OSLMI_ERR_UNSUPPORTED, "only the Rec709 color space is built in");
+        s->ss.colorspace = value;
+    } else if (n == "jit_cache") {
+        s->ss.cache_dir = value;
+    } else if (n == "include_dir") {
+        s->ss.include_dir = value;
+    } else {
+        return set_error(OSLMI_ERR_UNSUPPORTED, "unknown attribute " + n);
+    }
+    return OSLMI_OK;
+}
+
+int oslmi_load_memory_compiled_shader(oslmi_shadingsys* s, const char* shadername,
+                                      const char* oso_text)
+{
+    if (!s || !shadername || !oso_text)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_load_memory_compiled_shader: null argument");
+    auto m = std::make_unique<ShaderMaster>();
+    std::string err;
+    if (!parse_oso(oso_text, *m, err))
+        return set_error(OSLMI_ERR_INVALID, std::string(shadername) + ": " + err);
+    std::lock_guard<std::mutex> lock(s->ss.mu);
+    s->ss.masters[shadername] = std::move(m);
+    return OSLMI_OK;
+}
+
+oslmi_group* oslmi_shader_group_begin(oslmi_shadingsys* s, const char* groupname)
+{
+    if (!s)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(s->ss.mu);
+    s->ss.groups.push_back(std::make_unique<Group>());
+    Group* g = s->ss.groups.back().get();
+    g->name  = groupname ? groupname : "group";
+    return new oslmi_group { g };
+}
+
+int oslmi_parameter(oslmi_shadingsys* s, oslmi_group* gh, const char* name, const char* type,
+                    const void* value, int count)
+{
+    if (!s || !gh || !name || !type || !value || count < 1)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_parameter: bad argument");
+    Group& g = *gh->g;
+    if (g.ended)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_parameter: group already ended");
+    ParamValue pv;
+    if (!parse_type(type, pv.type))
+        return set_error(OSLMI_ERR_UNSUPPORTED, std::string("oslmi_parameter: type ") + type);
+    int ncomp = pv.type.base == kTriple ? 3 : pv.type.base == kMatrix ? 16 : 1;
+    if (pv.type.base == kString) {
+        const char* const* sv = (const char* const*)value;
+        for (int i = 0; i < count; ++i)
+            pv.s.push_back(sv[i]);
+    } else if (pv.type.base == kInt) {
+        const int* iv = (const int*)value;
+        for (int i = 0; i < count; ++i)
+            pv.i.push_back(iv[i]);
+    } else {
+        const float* fv = (const float*)value;
+        for (int i = 0; i < count * ncomp; ++i)
+            pv.f.push_back(fv[i]);
+    }
+    // A parameter applies to the next Shader() call (ShadingSystem
+    // semantics): values collect on a trailing layer that has no shader yet.
+    if (g.layers.empty() || g.layers.back().master != nullptr)
+        g.layers.push_back(Layer());
+    g.layers.back().inst[name] = pv;
+    return OSLMI_OK;
+}
+
+int oslmi_shader(oslmi_shadingsys* s, oslmi_group* gh, const char* usage, const char* shadername,
+                 const char* layername)
+{
+    if (!s || !gh || !shadername)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_shader: null argument");
+    Group& g = *gh->g;
+    auto it  = s->ss.masters.find(shadername);
+    if (it == s->ss.masters.end())
+        return set_error(OSLMI_ERR_INVALID, std::string("oslmi_shader: unknown shader ") + shadername);
+    (void)usage;
+    if (g.layers.empty() || g.layers.back().master != nullptr)
+        g.layers.push_back(Layer());
+    Layer& L = g.layers.back();
+    L.master = it->second.get();
+    L.name   = layername && *layername ? layername : ("layer" + std::to_string(g.layers.size() - 1));
+    for (auto& kv : L.inst)
+        if (L.master->find(kv.first) < 0)
+            return set_error(OSLMI_ERR_INVALID, "oslmi_shader: shader " + std::string(shadername)
+                                                    + " has no parameter " + kv.first);
+    return OSLMI_OK;
+}
+
+int oslmi_connect_shaders(oslmi_shadingsys* s, oslmi_group* gh, const char* srclayer,
+                          const char* srcparam, const char* dstlayer, const char* dstparam)
+{
+    if (!s || !gh || !srclayer || !srcparam || !dstlayer || !dstparam)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_connect_shaders: null argument");
+    Group& g = *gh->g;
+    int sl = g.find_layer(srclayer), dl = g.find_layer(dstlayer);
+    if (sl < 0 || dl < 0)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_connect_shaders: unknown layer");
+    int ss_ = g.layers[sl].master->find(srcparam), ds = g.layers[dl].master->find(dstparam);
+    if (ss_ < 0 || ds < 0)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_connect_shaders: unknown parameter");
+    if (sl >= dl)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_connect_shaders: source layer must come first");
+    g.conns.push_back({ sl, ss_, dl, ds });
+    return OSLMI_OK;
+}
+
+int oslmi_shader_group_end(oslmi_shadingsys* s, oslmi_group* gh)
+{
+    if (!s || !gh)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_shader_group_end: null argument");
+    Group& g = *gh->g;
+    if (!g.layers.empty() && g.layers.back().master == nullptr)
+        return set_error(OSLMI_ERR_INVALID, "parameters given after the last shader");
+    g.ended = true;
+    return OSLMI_OK;
+}
+
+int oslmi_group_outputs(oslmi_shadingsys* s, oslmi_group* gh, int n, const char* const* names)
+{
+    if (!s || !gh || n < 0 || (n > 0 && !names))
+        return set_error(OSLMI_ERR_INVALID, "oslmi_group_outputs: bad argument");
+    gh->g->output_names.assign(names, names + n);
+    gh->g->jitted = false;
+    return OSLMI_OK;
+}
+
+int oslmi_jit_group(oslmi_shadingsys* s, oslmi_group* gh)
+{
+    if (!s || !gh)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_jit_group: null argument");
+    Group& g = *gh->g;
+    std::lock_guard<std::mutex> lock(g.mu);
+    if (!g.ended)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_jit_group: group not ended");
+    return compile_group(s->ss, g);
+}
+
+int oslmi_compile_group(oslmi_shadingsys* s, oslmi_group* gh)
+{
+    if (!s || !gh)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_compile_group: null argument");
+    Group& g = *gh->g;
+    std::lock_guard<std::mutex> lock(g.mu);
+    if (!g.ended)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_compile_group: group not ended");
+    return compile_group(s->ss, g, false);
+}
+
+int oslmi_group_emit(oslmi_shadingsys* s, oslmi_group* gh, char* buf, int64_t bufsize)
+{
+    if (!s || !gh)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_group_emit: null argument");
+    Group& g = *gh->g;
+    std::lock_guard<std::mutex> lock(g.mu);
+    if (int rc = precompile_group(s->ss, g))
+        return rc;
+    if (buf && bufsize > 0) {
+        size_t n = std::min((size_t)bufsize - 1, g.source.size());
+        memcpy(buf, g.source.data(), n);
+        buf[n] = 0;
+    }
+    return (int)std::min<size_t>(g.source.size(), 0x7fffffff);
+}
+
+int oslmi_group_output_planes(oslmi_shadingsys* s, oslmi_group* gh)
+{
+    if (!s || !gh)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_group_output_planes: null argument");
+    return gh->g->jitted ? gh->g->output_planes : set_error(OSLMI_ERR_INVALID, "group not jitted");
+}
+
+double oslmi_group_jit_seconds(oslmi_group* gh) { return gh ? gh->g->jit_seconds : 0.0; }
+
+void oslmi_group_release(oslmi_group* gh) { delete gh; }
+
+int oslmi_execute_batch(oslmi_shadingsys* s, oslmi_group* gh, const oslmi_batched_globals* bg,
+                        float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
+                        void* stream)
+{
+    if (!s || !gh || !bg)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch: null argument");
+    Group& g = *gh->g;
+    if (!g.jitted) {
+        if (int rc = oslmi_jit_group(s, gh))
+            return rc;
+    }
+    if (bg->n < 0)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch: n < 0");
+    if (bg->n == 0)
+        return OSLMI_OK;
+    if (g.output_planes > 0 && !d_outputs)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch: group has outputs but d_outputs is NULL");
+    if (d_eval && (!g.surface_closure || !d_wo || !d_wi))
+        return set_error(OSLMI_ERR_INVALID,
+                         "oslmi_execute_batch: closure evaluation needs a surface closure and wo/wi");
+    GroupArgs A;
+    memset(&A, 0, sizeof(A));
+    A.n = bg->n;
+    for (int k = 0; k < kG_COUNT; ++k) {
+        A.g[k] = bg->planes[k];
+        for (int c = 0; c < 3; ++c)
+            A.gu[k][c] = bg->uniform[k][c];
+    }
+    for (int p = 0; p < g.output_planes; ++p)
+        A.out[p] = d_outputs + (int64_t)p * bg->n;
+    A.wo   = d_wo;
+    A.wi   = d_wi;
+    A.eval = d_eval;
+    if (g.uses_colorsys) {
+        const ColorSysDev* cs = nullptr;
+        if (int rc = device_colorsys(&cs))
+            return rc;
+        A.colorsys = cs;
+    }
+    void* args[] = { &A };
+    if (int rc = check_hip(hipModuleLaunchKernel(g.kernel, grid_for(bg->n), 1, 1, 256, 1, 1, 0,
+                                                 (hipStream_t)stream, args, nullptr),
+                           "hipModuleLaunchKernel"))
+        return rc;
+    return OSLMI_OK;
+}
+
+}  // extern "C"

@@ -12,6 +12,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#else
+// hiprtc provides the fixed-width integer types in an internal namespace
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::int64_t int64_t;
+typedef __hip_internal::uint64_t uint64_t;
 #endif
 
 #ifndef OSLMI_DEV

@@ -1,0 +1,193 @@
+"""ShadingSystem / ShaderGroup / BatchedExecutor over the C ABI.
+
+Mirrors the reference's host API (src/include/OSL/oslexec.h):
+LoadMemoryCompiledShader, ShaderGroupBegin/End, Parameter, Shader,
+ConnectShaders, attribute, and the batched executor's jit_group / execute.
+Inputs and outputs are torch CUDA tensors of SoA planes.
+"""
+import ctypes
+import os
+
+from ._capi import check, lib
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+SHADER_DIR = os.path.join(PKG, "shaders")
+CSRC = os.path.join(PKG, "csrc")
+JIT_CACHE = os.environ.get("OSLMI_JIT_CACHE", os.path.join(PKG, "_lib", "jitcache"))
+
+# global slot order of oslmi_batched_globals (include/oslmi.h)
+GLOBALS = ["P", "dPdx", "dPdy", "N", "Ng", "I", "u", "v", "dudx", "dudy", "dvdx", "dvdy",
+           "time", "dPdu", "dPdv", "Ps"]
+TRIPLES = {"P", "dPdx", "dPdy", "N", "Ng", "I", "dPdu", "dPdv", "Ps"}
+
+
+class BatchedGlobals(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64),
+                ("planes", ctypes.c_void_p * 16),
+                ("uniform", (ctypes.c_float * 3) * 16)]
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class ShaderGroup:
+    def __init__(self, ss, handle, name):
+        self.ss, self.h, self.name = ss, handle, name
+
+    def __del__(self):
+        try:
+            lib().oslmi_group_release(self.h)
+        except Exception:
+            pass
+
+
+class ShadingSystem:
+    def __init__(self, jit_cache=JIT_CACHE):
+        L = lib()
+        self.h = L.oslmi_shadingsys_create(CSRC.encode(),
+                                           jit_cache.encode() if jit_cache else None)
+        if not self.h:
+            raise RuntimeError("oslmi_shadingsys_create failed")
+
+    def __del__(self):
+        try:
+            lib().oslmi_shadingsys_destroy(self.h)
+        except Exception:
+            pass
+
+    def attribute(self, name, value):
+        check(lib().oslmi_shadingsys_attribute(self.h, name.encode(), str(value).encode()))
+
+    def LoadMemoryCompiledShader(self, shadername, oso_text):
+        check(lib().oslmi_load_memory_compiled_shader(self.h, shadername.encode(),
+                                                      oso_text.encode()))
+
+    def load_shader_file(self, shadername, path=None):
+        path = path or os.path.join(SHADER_DIR, shadername + ".oso")
+        with open(path) as f:
+            self.LoadMemoryCompiledShader(shadername, f.read())
+
+    def ShaderGroupBegin(self, name="group"):
+        h = lib().oslmi_shader_group_begin(self.h, name.encode())
+        return ShaderGroup(self, h, name)
+
+    def Parameter(self, group, name, typename, value):
+        if typename == "string":
+            vals = [value] if isinstance(value, str) else list(value)
+            arr = (ctypes.c_char_p * len(vals))(*[v.encode() for v in vals])
+            check(lib().oslmi_parameter(self.h, group.h, name.encode(), b"string",
+                                        ctypes.cast(arr, ctypes.c_void_p), len(vals)))
+            return
+        if typename == "int":
+            vals = [value] if isinstance(value, int) else list(value)
+            arr = (ctypes.c_int * len(vals))(*vals)
+            count = len(vals)
+        else:
+            vals = [float(value)] if isinstance(value, (int, float)) else [float(v) for v in value]
+            arr = (ctypes.c_float * len(vals))(*vals)
+            per = {"float": 1, "matrix": 16}.get(typename, 3)
+            count = max(1, len(vals) // per)
+        check(lib().oslmi_parameter(self.h, group.h, name.encode(), typename.encode(),
+                                    ctypes.cast(arr, ctypes.c_void_p), count))
+
+    def Shader(self, group, usage, shadername, layername):
+        check(lib().oslmi_shader(self.h, group.h, usage.encode(), shadername.encode(),
+                                 layername.encode()))
+
+    def ConnectShaders(self, group, srclayer, srcparam, dstlayer, dstparam):
+        check(lib().oslmi_connect_shaders(self.h, group.h, srclayer.encode(), srcparam.encode(),
+                                          dstlayer.encode(), dstparam.encode()))
+
+    def ShaderGroupEnd(self, group):
+        check(lib().oslmi_shader_group_end(self.h, group.h))
+
+    def set_outputs(self, group, names):
+        arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+        check(lib().oslmi_group_outputs(self.h, group.h, len(names), arr))
+
+    def emit(self, group):
+        """HIP source of the group's fused kernel (no device needed)."""
+        n = lib().oslmi_group_emit(self.h, group.h, None, 0)
+        check(n if n < 0 else 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().oslmi_group_emit(self.h, group.h, buf, n + 1)
+        return buf.value.decode()
+
+    def compile_group(self, group):
+        """Emit + hiprtc compile into the code-object cache (no device)."""
+        check(lib().oslmi_compile_group(self.h, group.h))
+        return lib().oslmi_group_jit_seconds(group.h)
+
+    def jit_group(self, group):
+        check(lib().oslmi_jit_group(self.h, group.h))
+        return lib().oslmi_group_jit_seconds(group.h)
+
+    def output_planes(self, group):
+        n = lib().oslmi_group_output_planes(self.h, group.h)
+        check(n if n < 0 else 0)
+        return n
+
+    def execute(self, group, n, globals_, outputs=None, wo=None, wi=None, eval_out=None,
+                stream=None):
+        """Batched execution over n points. globals_: dict name -> CUDA
+        tensor of planes (varying) or a tuple of floats (uniform)."""
+        self.bind(group, n, globals_, outputs, wo, wi, eval_out, stream)()
+
+    def bind(self, group, n, globals_, outputs=None, wo=None, wi=None, eval_out=None,
+             stream=None):
+        """Prepares the argument block once; returns a callable that
+        launches the batched execution (repeated launches reuse it)."""
+        bg = BatchedGlobals()
+        bg.n = n
+        for k, name in enumerate(GLOBALS):
+            v = globals_.get(name)
+            if v is None:
+                continue
+            if hasattr(v, "data_ptr"):
+                bg.planes[k] = v.data_ptr()
+            else:
+                vals = list(v) if hasattr(v, "__len__") else [v]
+                for c in range(3):
+                    bg.uniform[k][c] = float(vals[min(c, len(vals) - 1)])
+        import torch
+        sp = ctypes.c_void_p(stream.cuda_stream if stream is not None
+                             else torch.cuda.current_stream().cuda_stream)
+        fn = lib().oslmi_execute_batch
+        args = (self.h, group.h, ctypes.byref(bg), _ptr(outputs), _ptr(wo), _ptr(wi),
+                _ptr(eval_out), sp)
+        keep = (bg, globals_, outputs, wo, wi, eval_out)
+
+        def launch():
+            check(fn(*args))
+            return keep
+        return launch
+
+
+def build_fbm_cell(ss=None, outputs=("Cout",)):
+    """The single-layer fbm_cell group."""
+    ss = ss or ShadingSystem()
+    ss.load_shader_file("fbm_cell")
+    g = ss.ShaderGroupBegin("fbm_cell")
+    ss.Shader(g, "surface", "fbm_cell", "fbm")
+    ss.ShaderGroupEnd(g)
+    ss.set_outputs(g, list(outputs))
+    return ss, g
+
+
+def build_layered3(ss=None, outputs=()):
+    """The 3-layer layered3 group: pattern -> color -> surface."""
+    ss = ss or ShadingSystem()
+    for name, fn in (("c2_pattern", "layered3_pattern"), ("c2_color", "layered3_color"),
+                     ("c2_surface", "layered3_surface")):
+        ss.load_shader_file(name, os.path.join(SHADER_DIR, fn + ".oso"))
+    g = ss.ShaderGroupBegin("layered3")
+    ss.Shader(g, "shader", "c2_pattern", "pattern")
+    ss.Shader(g, "shader", "c2_color", "color")
+    ss.Shader(g, "surface", "c2_surface", "surface")
+    for p in ("fbm", "vn", "gab"):
+        ss.ConnectShaders(g, "pattern", p, "color", p)
+    ss.ConnectShaders(g, "color", "Cd", "surface", "Cd")
+    ss.ShaderGroupEnd(g)
+    ss.set_outputs(g, list(outputs))
+    return ss, g

@@ -1,0 +1,83 @@
+"""OSO reader, group building and the HIP emitter (no GPU needed: emission
+and the hiprtc compile run on the host)."""
+import os
+
+import pytest
+
+from openshadinglanguage_amd import OSLMIError
+from openshadinglanguage_amd import shadingsys as S
+
+pytest.importorskip("ctypes")
+
+
+def test_emit_fbm_cell():
+    ss, g = S.build_fbm_cell()
+    src = ss.emit(g)
+    assert "oslmi_group_main" in src
+    assert "perlin3<float>" in src and "intnoise3<true>" in src
+    # no derivatives anywhere in this group
+    assert "Df " not in src and "Dv " not in src
+
+
+def test_emit_layered3_derivative_analysis():
+    ss, g = S.build_layered3()
+    src = ss.emit(g)
+    # P of the pattern layer carries derivatives (gabor + filterwidth need them)
+    assert "Dv(ld3(A, 0, i), ld3(A, 1, i), ld3(A, 2, i))" in src
+    assert "simplex3_d(" in src and "vperlin3<Df>" in src and "gabor(gu_" in src
+    # the surface layer's noise needs no derivatives
+    assert "perlin3<float>" in src
+    assert "blackbody_rgb(s_bb" in src and "composite_eval" in src
+
+
+def test_hiprtc_compile_both_groups(tmp_path):
+    for build in (S.build_fbm_cell, S.build_layered3):
+        ss, g = build(S.ShadingSystem(jit_cache=str(tmp_path)))
+        ss.compile_group(g)
+    assert len([f for f in os.listdir(tmp_path) if f.endswith(".hsaco")]) == 2
+
+
+def test_bad_oso_is_rejected():
+    ss = S.ShadingSystem(jit_cache=None)
+    with pytest.raises(OSLMIError):
+        ss.LoadMemoryCompiledShader("bad", "OpenShadingLanguage 1.00\nshader bad\ncode ___main___\n\tadd x y z\n")
+    with pytest.raises(OSLMIError):
+        ss.LoadMemoryCompiledShader("bad2", "not an oso file\n")
+
+
+def test_unknown_param_and_layer_errors():
+    ss, _ = S.build_fbm_cell()
+    g = ss.ShaderGroupBegin("g2")
+    ss.Parameter(g, "nosuchparam", "float", 1.0)
+    with pytest.raises(OSLMIError):
+        ss.Shader(g, "surface", "fbm_cell", "a")
+    g3 = ss.ShaderGroupBegin("g3")
+    ss.Shader(g3, "surface", "fbm_cell", "a")
+    with pytest.raises(OSLMIError):
+        ss.ConnectShaders(g3, "a", "Cout", "missing", "c0")
+
+
+def test_parameter_override_reaches_code():
+    ss = S.ShadingSystem(jit_cache=None)
+    ss.load_shader_file("fbm_cell")
+    g = ss.ShaderGroupBegin("g")
+    ss.Parameter(g, "octaves", "int", 7)
+    ss.Parameter(g, "c0", "color", (0.5, 0.25, 1.0))
+    ss.Shader(g, "surface", "fbm_cell", "fbm")
+    ss.ShaderGroupEnd(g)
+    ss.set_outputs(g, ["Cout"])
+    src = ss.emit(g)
+    assert "int v0_0 = 7;" in src
+    assert "V3(0x1p-1f, 0x1p-2f, 0x1p+0f)" in src
+
+
+def test_unsupported_op_reports_error():
+    oso = ("OpenShadingLanguage 1.00\nshader s\nglobal point P\nlocal float x\n"
+           "code ___main___\n\tspline x P\n\tend\n")
+    ss = S.ShadingSystem(jit_cache=None)
+    ss.LoadMemoryCompiledShader("s", oso)
+    g = ss.ShaderGroupBegin("g")
+    ss.Shader(g, "surface", "s", "a")
+    ss.ShaderGroupEnd(g)
+    with pytest.raises(OSLMIError, match="unsupported op"):
+        ss.emit(g)
