@@ -136,6 +136,51 @@ int oslmi_shade_layered3(int64_t n, const float* d_P, const float* d_N, const fl
                          const float* d_wi, const float* dPdx, const float* dPdy, float* d_out,
                          float* d_aov, void* stream);
 
+/* ---- Layered MaterialX closure -----------------------------------------
+ * The closure layer(sheen_bsdf(N, sheen_albedo, sheen_roughness),
+ *   layer(dielectric_bsdf(N, 0, coat_refl_tint, coat_refr_tint, r, r, ior, "ggx"),
+ *         generalized_schlick_bsdf(N, 0, base_refl_tint, base_refr_tint, rb, rb,
+ *                                  F0, F90, base_exponent, "ggx")))
+ * (stdosl.h:522, :566, :606, :707) turned into lobes and evaluated/sampled
+ * like the renderer does for a camera-ray hit: process_bsdf_closure with
+ * evaluate_layer_opacity, CompositeBSDF::prepare/eval/sample
+ * (src/testrender/shading.cpp:1199-1272, :1448-1667, shading.h:319-393) over
+ * libbsdl's MTX lobes (libbsdl/include/BSDL/MTX/bsdf_{dielectric,schlick,
+ * sheen}_impl.h). Transmission tints must be zero (OSLMI_ERR_UNSUPPORTED
+ * otherwise). */
+typedef struct oslmi_layered_material {
+    float coat_refl_tint[3];
+    float coat_refr_tint[3];
+    float base_refl_tint[3];
+    float base_refr_tint[3];
+    float base_exponent;
+    float base_roughness; /* < 0: each point's coat roughness */
+    float sheen_albedo[3];
+} oslmi_layered_material;
+
+/* Per point: d_N, d_V (towards the viewer, -I), d_L (eval direction) 3
+ * planes each; d_params 9 planes (coat ior, coat roughness, sheen
+ * roughness, F0 rgb, F90 rgb); d_rnd 8 planes ((rx, ry) of samples 0..3).
+ * d_eval (NULL or 4 planes): CompositeBSDF::eval(V, L) as weight rgb, pdf.
+ * d_samples (NULL or 28 planes): CompositeBSDF::sample(V, rx, ry, 0) for
+ * k = 0..3 in planes 7k..7k+6 = wi xyz, weight rgb, pdf. */
+int oslmi_layered_closure(const oslmi_layered_material* m, int64_t n, const float* d_N,
+                          const float* d_V, const float* d_L, const float* d_params,
+                          const float* d_rnd, float* d_eval, float* d_samples, void* stream);
+
+/* CompositeBSDF::eval of the same material for queries in the shading
+ * frame (N = +z, V = (x, 0, z)): d_query 14 planes = L xyz, V x, V z, coat
+ * ior, coat roughness, sheen roughness, F0 rgb, F90 rgb; d_out 4 planes =
+ * weight rgb, pdf. */
+int oslmi_layered_eval_local(const oslmi_layered_material* m, int64_t n, const float* d_query,
+                             float* d_out, void* stream);
+
+/* Host copy of the energy-compensation tables (MiniMicrofacetGGX 16x16 then
+ * DielectricReflFront 32x16x16 = 8448 floats), baked on first use like
+ * libbsdl/src/genluts.cpp:128-192 and cached beside the library. No device
+ * needed. */
+int oslmi_bsdl_tables(float* out, int64_t count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,9 @@ def _declare(L):
         "oslmi_group_output_planes": ([P, P], I),
         "oslmi_group_jit_seconds": ([P], ctypes.c_double),
         "oslmi_execute_batch": ([P, P, P, P, P, P, P, P], I),
+        "oslmi_layered_closure": ([P, I64, P, P, P, P, P, P, P, P], I),
+        "oslmi_layered_eval_local": ([P, I64, P, P, P], I),
+        "oslmi_bsdl_tables": ([P, I64], I),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -62,6 +65,25 @@ def lib():
         _declare(L)
         _lib = L
     return _lib
+
+
+class LayeredMaterial(ctypes.Structure):
+    """oslmi_layered_material (include/oslmi.h)."""
+    _fields_ = [("coat_refl_tint", ctypes.c_float * 3),
+                ("coat_refr_tint", ctypes.c_float * 3),
+                ("base_refl_tint", ctypes.c_float * 3),
+                ("base_refr_tint", ctypes.c_float * 3),
+                ("base_exponent", ctypes.c_float),
+                ("base_roughness", ctypes.c_float),
+                ("sheen_albedo", ctypes.c_float * 3)]
+
+    def as_list(self):
+        """The 17 floats in declaration order."""
+        out = []
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out.extend(list(v) if hasattr(v, "__len__") else [v])
+        return out
 
 
 def check(rc):

@@ -49,7 +49,25 @@ def build(verbose=False, jobs=8):
         "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    bake_tables()
     return LIB
+
+
+def bake_tables():
+    """Bakes libbsdl's energy-compensation tables with the freshly built
+    library (host code, no device) into _lib/bsdl_luts.bin, where the
+    library loads them from; the build-time step libbsdl's genluts is."""
+    import ctypes
+    path = os.path.join(OUT_DIR, "bsdl_luts.bin")
+    if os.path.exists(path):
+        os.remove(path)
+    code = ("import ctypes,sys; L=ctypes.CDLL(sys.argv[1]); b=(ctypes.c_float*8448)(); "
+            "sys.exit(L.oslmi_bsdl_tables(b, ctypes.c_int64(8448)))")
+    # a child process, so this process never maps the HIP runtime
+    subprocess.check_call([sys.executable, "-c", code, LIB])
+    if not os.path.exists(path):
+        raise RuntimeError("table bake did not write %s" % path)
+    del ctypes
 
 
 def _wait(item):

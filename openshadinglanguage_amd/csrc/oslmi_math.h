@@ -87,6 +87,82 @@ OSLMI_DEV float fcbrt(float x)
     return x < 0.0f ? -y : y;
 }
 
+// log2(x), finite x > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s), s = (m - 1) / (m + 1), series to s^9.
+OSLMI_DEV float flog2(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    int e      = 0;
+    if (u < 0x00800000u) {
+        x = x * 8388608.0f;
+        u = __float_as_uint(x);
+        e = -23;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = __uint_as_float((u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421353816986083984375f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    float s  = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p  = 1.0f / 9.0f;
+    p        = __fmaf_rn(p, s2, 1.0f / 7.0f);
+    p        = __fmaf_rn(p, s2, 1.0f / 5.0f);
+    p        = __fmaf_rn(p, s2, 1.0f / 3.0f);
+    p        = __fmaf_rn(p, s2, 1.0f);
+    float ln = 2.0f * s * p;
+    return __fmaf_rn(ln, 1.44269502162933349609375f, (float)e);
+}
+
+// 2^t: t = k + f, k = rint(t), 2^f by the Taylor series of e^(f ln2) to
+// degree 7, 2^k in two exact scalings.
+OSLMI_DEV float fexp2(float t)
+{
+    if (!(t == t))
+        return t;
+    if (t >= 128.0f)
+        return __uint_as_float(0x7f800000u);
+    if (t < -150.0f)
+        return 0.0f;
+    float k  = rintf(t);
+    float f  = t - k;
+    float p  = 1.5252733804059840e-05f;
+    p        = __fmaf_rn(p, f, 1.5403530393381608e-04f);
+    p        = __fmaf_rn(p, f, 1.3333558146428443e-03f);
+    p        = __fmaf_rn(p, f, 9.6181291076284772e-03f);
+    p        = __fmaf_rn(p, f, 5.5504108664821580e-02f);
+    p        = __fmaf_rn(p, f, 2.4022650695910071e-01f);
+    p        = __fmaf_rn(p, f, 6.9314718055994531e-01f);
+    p        = __fmaf_rn(p, f, 1.0f);
+    int ki   = (int)k;
+    int k1   = ki / 2;
+    int k2   = ki - k1;
+    float s1 = __uint_as_float((uint32_t)(k1 + 127) << 23);
+    float s2 = __uint_as_float((uint32_t)(k2 + 127) << 23);
+    return (p * s1) * s2;
+}
+
+// x^y with fast_safe_pow's special cases (y == 0 -> 1, x == 0 -> 0,
+// y == 1 -> x, y == 2 -> min(x*x, FLT_MAX)); x < 0 -> 0; else
+// 2^(log2(x) y).
+OSLMI_DEV float fsafe_pow(float x, float y)
+{
+    if (y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f)
+        return 0.0f;
+    if (y == 1.0f)
+        return x;
+    if (y == 2.0f) {
+        float x2 = x * x;
+        return x2 < 3.40282346638528859812e+38f ? x2 : 3.40282346638528859812e+38f;
+    }
+    if (x < 0.0f)
+        return 0.0f;
+    return fexp2(flog2(x) * y);
+}
+
 // Dual2 exp and cos (dual.h:830-836, :1122-1127): f(a) with f'(a) * da.
 OSLMI_DEV Df dexp(const Df& a)
 {

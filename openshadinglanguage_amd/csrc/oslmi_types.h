@@ -22,24 +22,28 @@ typedef __hip_internal::uint64_t uint64_t;
 #ifndef OSLMI_DEV
 #define OSLMI_DEV __device__ __forceinline__
 #endif
+// Host and device: code the library also runs on the CPU (table bakes).
+#ifndef OSLMI_HD
+#define OSLMI_HD __host__ __device__ __forceinline__
+#endif
 
 namespace oslmi {
 
 struct V3 {
     float x, y, z;
-    OSLMI_DEV V3() : x(0.f), y(0.f), z(0.f) {}
-    OSLMI_DEV V3(float a, float b, float c) : x(a), y(b), z(c) {}
-    OSLMI_DEV explicit V3(float a) : x(a), y(a), z(a) {}
+    OSLMI_HD V3() : x(0.f), y(0.f), z(0.f) {}
+    OSLMI_HD V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    OSLMI_HD explicit V3(float a) : x(a), y(a), z(a) {}
 };
-OSLMI_DEV V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
-OSLMI_DEV V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
-OSLMI_DEV V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
-OSLMI_DEV V3 operator*(V3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
-OSLMI_DEV V3 operator*(float s, V3 a) { return V3(s * a.x, s * a.y, s * a.z); }
-OSLMI_DEV V3 operator*(V3 a, V3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
-OSLMI_DEV V3 operator/(V3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
-OSLMI_DEV float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-OSLMI_DEV V3 cross(V3 a, V3 b)
+OSLMI_HD V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+OSLMI_HD V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+OSLMI_HD V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
+OSLMI_HD V3 operator*(V3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+OSLMI_HD V3 operator*(float s, V3 a) { return V3(s * a.x, s * a.y, s * a.z); }
+OSLMI_HD V3 operator*(V3 a, V3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+OSLMI_HD V3 operator/(V3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+OSLMI_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+OSLMI_HD V3 cross(V3 a, V3 b)
 {
     return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }

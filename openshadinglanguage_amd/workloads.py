@@ -54,3 +54,79 @@ def layered3_inputs(res=1024, seed=2, n=None):
     return {"P": P32, "N": P32.copy(), "wo": f(wo), "wi": f(wi),
             "dPdx": np.array([8.0 / 1024, 0.0, 0.0], np.float32),
             "dPdy": np.array([0.0, 8.0 / 1024, 0.0], np.float32)}
+
+
+# Layered closure material of the closure workload: white reflection tints,
+# no transmission, Schlick exponent 5, base roughness = coat roughness,
+# white sheen (oslmi_layered_material, include/oslmi.h).
+LAYERED_MATERIAL = {
+    "coat_refl_tint": (1.0, 1.0, 1.0), "coat_refr_tint": (0.0, 0.0, 0.0),
+    "base_refl_tint": (1.0, 1.0, 1.0), "base_refr_tint": (0.0, 0.0, 0.0),
+    "base_exponent": 5.0, "base_roughness": -1.0, "sheen_albedo": (1.0, 1.0, 1.0),
+}
+
+
+def layered_material():
+    from ._capi import LayeredMaterial
+    m = LayeredMaterial()
+    for k, v in LAYERED_MATERIAL.items():
+        if isinstance(v, tuple):
+            getattr(m, k)[:] = list(v)
+        else:
+            setattr(m, k, v)
+    return m
+
+
+def _unit_sphere(rng, n):
+    z = rng.uniform(-1.0, 1.0, size=n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, size=n)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z])
+
+
+def _cosine_about(rng, N):
+    """Cosine-distributed directions about the unit vectors N (Malley)."""
+    n = N.shape[1]
+    r = np.sqrt(rng.uniform(0.0, 1.0, size=n))
+    phi = rng.uniform(0.0, 2.0 * np.pi, size=n)
+    x, y = r * np.cos(phi), r * np.sin(phi)
+    z = np.sqrt(np.maximum(0.0, 1.0 - r * r))
+    # orthonormal basis about N (Duff et al.)
+    s = np.where(N[2] >= 0, 1.0, -1.0)
+    a = -1.0 / (s + N[2])
+    b = N[0] * N[1] * a
+    T = np.stack([1.0 + s * N[0] * N[0] * a, s * b, -s * N[0]])
+    B = np.stack([b, s + N[1] * N[1] * a, -N[1]])
+    return x * T + y * B + z * N
+
+
+def layered_closure_inputs(n=1 << 21, seed=4):
+    """Closure workload: N uniform on the sphere, the view direction V
+    cosine-distributed about N, the eval direction L uniform on the sphere,
+    coat ior U[1.3,1.8], coat roughness U[0.05,1], sheen roughness U[0.1,1],
+    F0, F90 U[0,1]^3, and 4 (rx, ry) pairs U[0,1)^2 per point."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = _unit_sphere(rng, n)
+    V = _cosine_about(rng, N)
+    L = _unit_sphere(rng, n)
+    prm = np.concatenate([rng.uniform(1.3, 1.8, size=(1, n)), rng.uniform(0.05, 1.0, size=(1, n)),
+                          rng.uniform(0.1, 1.0, size=(1, n)), rng.uniform(0.0, 1.0, size=(6, n))])
+    rnd = rng.uniform(0.0, 1.0, size=(8, n))
+    f = lambda a: np.ascontiguousarray(a.astype(np.float32))  # noqa: E731
+    rnd32 = f(rnd)
+    rnd32[rnd32 >= 1.0] = np.float32(0.99999994)
+    return {"N": f(N), "V": f(V), "L": f(L), "params": f(prm), "rnd": rnd32}
+
+
+def surrogate_queries(n, seed=6):
+    """Closure queries in the shading frame (N = +z), 14 float32 planes:
+    L xyz uniform on the sphere, V = (sqrt(1 - mu^2), 0, mu) with mu
+    cosine-distributed (mu = sqrt(xi)), then the closure workload's
+    parameter ranges (coat ior, coat roughness, sheen roughness, F0, F90)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = _unit_sphere(rng, n)
+    mu = np.sqrt(rng.uniform(0.0, 1.0, size=n))
+    V = np.stack([np.sqrt(np.maximum(0.0, 1.0 - mu * mu)), mu])
+    prm = np.concatenate([rng.uniform(1.3, 1.8, size=(1, n)), rng.uniform(0.05, 1.0, size=(1, n)),
+                          rng.uniform(0.1, 1.0, size=(1, n)), rng.uniform(0.0, 1.0, size=(6, n))])
+    return np.ascontiguousarray(np.concatenate([L, V, prm]).astype(np.float32))

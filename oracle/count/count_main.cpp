@@ -2,6 +2,12 @@
 // recompiled with counted scalar types. Exposes per-workload op totals.
 #include "counted.h"
 
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+
 thread_local uint64_t ocount::g_ops = 0;
 
 #define OREF_COUNTING 1
@@ -12,6 +18,8 @@ thread_local uint64_t ocount::g_ops = 0;
 #include "../color.cpp"
 #include "../closure.cpp"
 #include "../shade.cpp"
+#include "../mtx.cpp"
+#include "../mtx_api.cpp"
 #undef float
 #undef uint32_t
 

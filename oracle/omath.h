@@ -110,6 +110,87 @@ inline float ofast_expm1f(float x)
     return ofast_expf(x) - 1.0f;
 }
 
+// log2(x) for finite x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716, series to s^9
+// (truncation < 3e-9 relative), then e + ln(m) log2(e).
+inline float ofast_log2f(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    int e = 0;
+    if (u < 0x00800000u) {  // subnormal: scale into the normal range
+        x = x * 8388608.0f;
+        std::memcpy(&u, &x, 4);
+        e = -23;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = bits_f((u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421353816986083984375f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    float s  = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p  = 1.0f / 9.0f;
+    p        = fmaf(p, s2, 1.0f / 7.0f);
+    p        = fmaf(p, s2, 1.0f / 5.0f);
+    p        = fmaf(p, s2, 1.0f / 3.0f);
+    p        = fmaf(p, s2, 1.0f);
+    float ln = 2.0f * s * p;
+    return fmaf(ln, 1.44269502162933349609375f, (float)e);
+}
+
+// 2^t: t = k + f with k = rint(t) (f exact, |f| <= 1/2), 2^f by the Taylor
+// series of e^(f ln2) to degree 7 (truncation < 6e-9), 2^k applied in two
+// exact scalings.
+inline float ofast_exp2f(float t)
+{
+    if (!(t == t))
+        return t;
+    if (t >= 128.0f)
+        return bits_f(0x7f800000u);
+    if (t < -150.0f)
+        return 0.0f;
+    float k = rintf(t);
+    float f = t - k;
+    float p = 1.5252733804059840e-05f;  // ln2^7 / 7!
+    p       = fmaf(p, f, 1.5403530393381608e-04f);  // ln2^6 / 6!
+    p       = fmaf(p, f, 1.3333558146428443e-03f);  // ln2^5 / 5!
+    p       = fmaf(p, f, 9.6181291076284772e-03f);  // ln2^4 / 4!
+    p       = fmaf(p, f, 5.5504108664821580e-02f);  // ln2^3 / 3!
+    p       = fmaf(p, f, 2.4022650695910071e-01f);  // ln2^2 / 2!
+    p       = fmaf(p, f, 6.9314718055994531e-01f);  // ln2
+    p       = fmaf(p, f, 1.0f);
+    int ki  = (int)k;
+    int k1  = ki / 2;
+    int k2  = ki - k1;
+    float s1 = bits_f((uint32_t)(k1 + 127) << 23);
+    float s2 = bits_f((uint32_t)(k2 + 127) << 23);
+    return (p * s1) * s2;
+}
+
+// x^y with the special cases of OpenImageIO's fast_safe_pow (y == 0 -> 1,
+// x == 0 -> 0, y == 1 -> x, y == 2 -> min(x*x, FLT_MAX)); x < 0 returns 0
+// (the closure code calls it with x >= 0 only); otherwise
+// 2^(log2(x) * y).
+inline float ofast_safe_powf(float x, float y)
+{
+    if (y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f)
+        return 0.0f;
+    if (y == 1.0f)
+        return x;
+    if (y == 2.0f) {
+        const float x2   = x * x;
+        const float fmax = 3.40282346638528859812e+38f;
+        return x2 < fmax ? x2 : fmax;
+    }
+    if (x < 0.0f)
+        return 0.0f;
+    return ofast_exp2f(ofast_log2f(x) * y);
+}
+
 // Cube root: bit-level initial guess, then three Newton steps
 // y <- y - (y^3 - x) / (3 y^2) for x > 0 (sign restored for x < 0).
 inline float ofast_cbrtf(float x)

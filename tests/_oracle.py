@@ -137,3 +137,78 @@ def omath(fn, x):
     out = np.zeros_like(x)
     L.oref_math(["exp", "sin", "cos", "cbrt"].index(fn), _p(x), _p(out), x.size)
     return out
+
+
+# ---- layered MaterialX closure (oracle/mtx.cpp) ----------------------------
+MTX_FN = {"ggx_D": (0, 5, 1), "ggx_G1": (1, 5, 1), "ggx_G2_G1": (2, 8, 1),
+          "ggx_D_refl_D": (3, 5, 1), "vndf": (4, 7, 3), "bvndf": (5, 7, 3),
+          "unit_disc": (6, 2, 2), "uniform_hemisphere": (7, 2, 3), "fresnel": (8, 2, 1),
+          "emiss": (9, 4, 1), "ck_albedo": (10, 2, 1), "ck_D": (11, 4, 1), "pow": (12, 2, 1),
+          "log2": (13, 1, 1), "exp2": (14, 1, 1)}
+
+
+def _mtx_setup():
+    L = lib()
+    P, I64 = ctypes.c_void_p, ctypes.c_int64
+    L.oref_mtx_layered.argtypes = [P, P, P, P, P, P, P, P, I64, ctypes.c_int]
+    L.oref_mtx_eval_local.argtypes = [P, P, P, I64, ctypes.c_int]
+    L.oref_mtx_weights.argtypes = [P, P, P, P, P, P, I64]
+    L.oref_mtx_luts.argtypes = [P]
+    L.oref_mtx_bake.argtypes = [P, ctypes.c_int]
+    L.oref_mtx_fn.argtypes = [ctypes.c_int, P, P, I64]
+    L.oref_mtx_fn.restype = ctypes.c_int
+    return L
+
+
+def mtx_fn(name, rows):
+    """Building block `name` on AoS rows [n, width_in] -> [n, width_out]."""
+    fn, win, wout = MTX_FN[name]
+    rows = np.ascontiguousarray(np.asarray(rows, np.float32).reshape(-1, win))
+    out = np.zeros((rows.shape[0], wout), np.float32)
+    assert _mtx_setup().oref_mtx_fn(fn, _p(rows), _p(out), rows.shape[0]) == 0
+    return out
+
+
+def mtx_tables():
+    out = np.zeros(8448, np.float32)
+    _mtx_setup().oref_mtx_luts(_p(out))
+    return out
+
+
+def mtx_bake(nthreads=8):
+    out = np.zeros(8448, np.float32)
+    _mtx_setup().oref_mtx_bake(_p(out), nthreads)
+    return out
+
+
+def mtx_layered(mat, N, V, L, params, rnd, nthreads=1, eval_=True, samples=True):
+    """Returns (eval [4, n] or None, samples [28, n] or None)."""
+    Lb = _mtx_setup()
+    m = np.ascontiguousarray(mat, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (N, V, L, params, rnd)]
+    n = a[0].shape[1]
+    ev = np.zeros((4, n), np.float32) if eval_ else None
+    sp = np.zeros((28, n), np.float32) if samples else None
+    Lb.oref_mtx_layered(_p(m), *[_p(x) for x in a], _p(ev) if eval_ else None,
+                        _p(sp) if samples else None, n, nthreads)
+    return ev, sp
+
+
+def mtx_eval_local(mat, q, nthreads=1):
+    Lb = _mtx_setup()
+    m = np.ascontiguousarray(mat, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    out = np.zeros((4, q.shape[1]), np.float32)
+    Lb.oref_mtx_eval_local(_p(m), _p(q), _p(out), q.shape[1], nthreads)
+    return out
+
+
+def mtx_weights(mat, N, V, params):
+    Lb = _mtx_setup()
+    m = np.ascontiguousarray(mat, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (N, V, params)]
+    n = a[0].shape[1]
+    w9 = np.zeros((9, n), np.float32)
+    p3 = np.zeros((3, n), np.float32)
+    Lb.oref_mtx_weights(_p(m), *[_p(x) for x in a], _p(w9), _p(p3), n)
+    return w9, p3
