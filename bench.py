@@ -74,6 +74,83 @@ def cpu_baseline(name, inputs, nthreads, passes):
                       % (passes, n, nthreads, dt)}
 
 
+def _timed(step, stream, steps, warmup):
+    """(ms per launch from HIP events on `stream`, wall ms per step)."""
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return ev0.elapsed_time(ev1) / steps, wall * 1e3 / steps
+
+
+# closure workload: 2M points, per point 1 eval + 4 samples; bytes per point:
+# N, V, L 36 + params 36 + rnd 32 in; eval 16 + samples 4 x 28 out
+CLOSURE_POINTS = 1 << 21
+CLOSURE_BYTES_PER_POINT = 36 + 36 + 32 + 16 + 112
+
+
+def bench_closure(steps, warmup, cpu, cpu_passes):
+    """The layered MaterialX closure over the 2M-point closure workload."""
+    import torch
+    from openshadinglanguage_amd import check, lib
+    from openshadinglanguage_amd import workloads as W
+    dev = torch.device("cuda:0")
+    w = W.layered_closure_inputs(CLOSURE_POINTS, seed=4)
+    n = w["N"].shape[1]
+    t = {k: torch.from_numpy(w[k]).to(dev) for k in ("N", "V", "L", "params", "rnd")}
+    ev = torch.empty((4, n), device=dev)
+    smp = torch.empty((28, n), device=dev)
+    m = W.layered_material()
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    p = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+
+    def step():
+        check(lib().oslmi_layered_closure(ctypes.byref(m), n, p(t["N"]), p(t["V"]), p(t["L"]),
+                                          p(t["params"]), p(t["rnd"]), p(ev), p(smp), sp))
+    kernel_ms, ms_per_step = _timed(step, stream, steps, warmup)
+    ops = _ops_per_point("closure")
+    achieved = ops * n / (kernel_ms / 1e3)
+    res = {"workload": "layer(sheen, layer(dielectric coat, generalized-Schlick base)): "
+                       "1 eval + 4 samples per point", "points": n,
+           "Mpts_per_s": round(n / (ms_per_step / 1e3) / 1e6, 2),
+           "closure_evals_per_s": round(5 * n / (ms_per_step / 1e3), 1),
+           "ms_per_step": round(ms_per_step, 4),
+           "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                        "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
+                        "frac": round(achieved / VALU_PEAK_OPS, 4), "traffic": None,
+                        "ops_per_point": round(ops, 1), "kernel_ms": round(kernel_ms, 4),
+                        "hbm_algorithmic_GBps": round(CLOSURE_BYTES_PER_POINT * n
+                                                      / (kernel_ms / 1e3) / 1e9, 2)}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        nthreads = min(16, os.cpu_count() or 1)
+        mat = m.as_list()
+        fn = lambda: O.mtx_layered(mat, w["N"], w["V"], w["L"], w["params"], w["rnd"],  # noqa
+                                   nthreads=nthreads)
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(cpu_passes):
+            fn()
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(n * cpu_passes / dt / 1e6, 3), "unit": "Mpts/s",
+                               "cores": nthreads, "kind": "port",
+                               "sample": "%d pass(es) over the full %d-point workload, %d "
+                                         "threads, %.1f s wall" % (cpu_passes, n, nthreads, dt)}
+        res["gpu_over_cpu"] = round(res["Mpts_per_s"] / res["cpu_baseline"]["value"], 1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +161,8 @@ def main():
                     help="jit: group built from OSO and lowered by the emitter (default); "
                          "prebuilt: the ahead-of-time lowering")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the layered-closure measurement")
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
@@ -198,6 +277,9 @@ def main():
         cpu_inp = inp if name == "fbm_cell" else inp
         line["cpu_baseline"] = cpu_baseline(name, cpu_inp, nthreads, args.cpu_passes)
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    if not args.no_closure:
+        line["closure"] = bench_closure(max(5, args.steps // 2), args.warmup, not args.no_cpu,
+                                        args.cpu_passes)
     print(json.dumps(line), flush=True)
 
 

@@ -181,6 +181,52 @@ int oslmi_layered_eval_local(const oslmi_layered_material* m, int64_t n, const f
  * needed. */
 int oslmi_bsdl_tables(float* out, int64_t count);
 
+/* ---- Learned surrogate of the layered closure --------------------------
+ * No counterpart in the reference: a 14 -> 4 x 64 SiLU -> 4 MLP that
+ * approximates oslmi_layered_eval_local (inputs: its 14 query planes;
+ * outputs: log eval weight rgb and log pdf), trained and evaluated with
+ * BF16 MFMA / FP32 accumulation kernels. Parameters: 16672 floats, row-major
+ * W1[64][32] (input columns 14..31 zero), b1[64], W2..W4[64][64], b2..b4,
+ * W5[32][64] (output rows 4..31 zero), b5[32]. */
+typedef struct oslmi_surrogate oslmi_surrogate;
+/* max_batch: the largest training batch, a multiple of 2048. Allocates
+ * device workspaces on the current device. */
+oslmi_surrogate* oslmi_surrogate_create(int64_t max_batch);
+void oslmi_surrogate_destroy(oslmi_surrogate* s);
+int oslmi_surrogate_param_count(void);
+/* Input normalisation (x - offset) * scale (14 each), output normalisation
+ * (y - mean) / std (4 each), and the training domain of the 9 material
+ * parameters (queries outside it are answered by the exact closure). */
+int oslmi_surrogate_set_normalization(oslmi_surrogate* s, const float* in_offset,
+                                      const float* in_scale, const float* out_mean,
+                                      const float* out_std, const float* domain_lo,
+                                      const float* domain_hi);
+int oslmi_surrogate_set_params(oslmi_surrogate* s, const float* params, int64_t count);
+int oslmi_surrogate_get_params(oslmi_surrogate* s, float* params, int64_t count);
+/* Gradient of the mean squared error of the normalised outputs over one
+ * batch (d_query 14 planes, d_target 4 planes = exact weight rgb, pdf; both
+ * with leading dimension batch, batch a multiple of 64) into host grads
+ * (16672 floats); *loss (optional) receives the loss. Synchronizes. */
+int oslmi_surrogate_grads(oslmi_surrogate* s, int64_t batch, const float* d_query,
+                          const float* d_target, float* grads, float* loss);
+/* One Adam step (step counts from 1) on one batch, asynchronous on stream. */
+int oslmi_surrogate_train_step(oslmi_surrogate* s, int64_t batch, const float* d_query,
+                               const float* d_target, float lr, float beta1, float beta2,
+                               float eps, int step, void* stream);
+/* Routing table: 64 cells (8 coat-roughness bins x 8 bins of the cosine
+ * between L and the mirror direction of V); a non-zero cell sends its
+ * queries to the exact closure. */
+int oslmi_surrogate_set_routing(oslmi_surrogate* s, const unsigned char* cells, int count);
+/* Answers n queries (d_query 14 planes) into d_out (4 planes: weight rgb,
+ * pdf). With m non-NULL, queries outside the training domain or in routed
+ * cells are listed in d_fallback_index / *d_fallback_count (device, n
+ * ints / 1 int) and evaluated exactly with material m; with m NULL the
+ * network answers every in-domain query and the others get NaN. Queries
+ * with V.z > 0 and L.z <= 0 get zeros (exact). */
+int oslmi_surrogate_infer(oslmi_surrogate* s, const oslmi_layered_material* m, int64_t n,
+                          const float* d_query, float* d_out, int* d_fallback_index,
+                          int* d_fallback_count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,17 @@ def _declare(L):
         "oslmi_layered_closure": ([P, I64, P, P, P, P, P, P, P, P], I),
         "oslmi_layered_eval_local": ([P, I64, P, P, P], I),
         "oslmi_bsdl_tables": ([P, I64], I),
+        "oslmi_surrogate_create": ([I64], P),
+        "oslmi_surrogate_destroy": ([P], None),
+        "oslmi_surrogate_param_count": ([], I),
+        "oslmi_surrogate_set_normalization": ([P, P, P, P, P, P, P], I),
+        "oslmi_surrogate_set_params": ([P, P, I64], I),
+        "oslmi_surrogate_get_params": ([P, P, I64], I),
+        "oslmi_surrogate_grads": ([P, I64, P, P, P, P], I),
+        "oslmi_surrogate_train_step": ([P, I64, P, P, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_float, I, P], I),
+        "oslmi_surrogate_set_routing": ([P, P, I], I),
+        "oslmi_surrogate_infer": ([P, P, I64, P, P, P, P, P], I),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

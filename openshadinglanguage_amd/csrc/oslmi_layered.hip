@@ -592,6 +592,30 @@ __global__ __launch_bounds__(256) void k_layered(Material mat, const float* __re
 // Exact evaluation of surrogate queries in the local frame (N = +z,
 // wo = (x, 0, z)): q 14 planes = wi xyz, wo x z, coat ior, coat roughness,
 // sheen roughness, F0 rgb, F90 rgb; out 4 planes = weight rgb, pdf.
+OSLMI_DEV void eval_query(const Material& mat, const float* T, const float* __restrict__ q,
+                          int64_t n, int64_t i, float* __restrict__ out)
+{
+    const float lz = q[2 * n + i], vz = q[4 * n + i];
+    if (vz > 0.0f && lz <= 0.0f) {
+        // N = +z is the visible normal and every lobe is reflection-only:
+        // the composite's weight and pdf are exactly zero
+        out[i] = out[n + i] = out[2 * n + i] = out[3 * n + i] = 0.0f;
+        return;
+    }
+    const float* p = q + 5 * n;
+    Layered L;
+    layered_build(&L, mat, V3(0.0f, 0.0f, 1.0f), V3(q[3 * n + i], 0.0f, vz), p[i], p[n + i],
+                  p[2 * n + i], V3(p[3 * n + i], p[4 * n + i], p[5 * n + i]),
+                  V3(p[6 * n + i], p[7 * n + i], p[8 * n + i]), T);
+    V3 w;
+    float pdf;
+    layered_eval(L, V3(q[i], q[n + i], lz), &w, &pdf);
+    out[i]         = w.x;
+    out[n + i]     = w.y;
+    out[2 * n + i] = w.z;
+    out[3 * n + i] = pdf;
+}
+
 __global__ __launch_bounds__(256) void k_layered_local(Material mat,
                                                        const float* __restrict__ lut,
                                                        const float* __restrict__ q,
@@ -602,19 +626,32 @@ __global__ __launch_bounds__(256) void k_layered_local(Material mat,
         T[k] = lut[k];
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float* p = q + 5 * n;
-        Layered L;
-        layered_build(&L, mat, V3(0.0f, 0.0f, 1.0f), V3(q[3 * n + i], 0.0f, q[4 * n + i]),
-                      p[i], p[n + i], p[2 * n + i], V3(p[3 * n + i], p[4 * n + i], p[5 * n + i]),
-                      V3(p[6 * n + i], p[7 * n + i], p[8 * n + i]), T);
-        V3 w;
-        float pdf;
-        layered_eval(L, V3(q[i], q[n + i], q[2 * n + i]), &w, &pdf);
-        out[i]         = w.x;
-        out[n + i]     = w.y;
-        out[2 * n + i] = w.z;
-        out[3 * n + i] = pdf;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        eval_query(mat, T, q, n, i, out);
+}
+
+// The same over the query indices idx[0 .. *count) (count written on the
+// device by an earlier kernel on the stream; every index < n).
+__global__ __launch_bounds__(256) void k_layered_local_indexed(Material mat,
+                                                               const float* __restrict__ lut,
+                                                               const float* __restrict__ q,
+                                                               int64_t n,
+                                                               const int* __restrict__ idx,
+                                                               const int* __restrict__ count,
+                                                               float* __restrict__ out)
+{
+    __shared__ float T[kLutFloats];
+    const int64_t cnt = *count;
+    if ((int64_t)blockIdx.x * blockDim.x >= cnt)
+        return;
+    for (int k = threadIdx.x; k < kLutFloats; k += blockDim.x)
+        T[k] = lut[k];
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
+        const int64_t qi = idx[i];
+        if (qi >= 0 && qi < n)
+            eval_query(mat, T, q, n, qi, out);
     }
 }
 
@@ -645,6 +682,23 @@ static int material_of(const oslmi_layered_material* m, mtx::Material* out, cons
     out->base_roughness = m->base_roughness;
     out->sheen_albedo   = V3(m->sheen_albedo[0], m->sheen_albedo[1], m->sheen_albedo[2]);
     return OSLMI_OK;
+}
+
+// Exact evaluation of the queries listed in d_index[0 .. *d_count) (the
+// surrogate's fallback list); the grid covers up to n listed queries.
+int layered_eval_local_indexed(const oslmi_layered_material* m, int64_t n, const float* d_query,
+                               const int* d_index, const int* d_count, float* d_out,
+                               hipStream_t stream)
+{
+    mtx::Material mat;
+    if (int rc = material_of(m, &mat, "surrogate fallback"))
+        return rc;
+    const float* lut = nullptr;
+    if (int rc = mtx::device_tables(&lut))
+        return rc;
+    hipLaunchKernelGGL(mtx::k_layered_local_indexed, dim3(layered_grid(n)), dim3(256), 0, stream,
+                       mat, lut, d_query, n, d_index, d_count, d_out);
+    return check_launch("surrogate fallback");
 }
 
 }  // namespace oslmi

@@ -53,6 +53,30 @@ def main():
                        "sample": "32768 random points of the 1024x1024 workload (seed 0)"}
     ref = O.shade_c2(sub["P"], sub["N"], sub["wo"], sub["wi"], sub["dPdx"], sub["dPdy"])
     assert np.array_equal(out, ref), "counting build differs from oracle"
+    # layered closure: 1 eval + 4 samples per point, and eval-only queries
+    L.oref_mtx_layered.argtypes = [P] * 8 + [ctypes.c_int64, ctypes.c_int]
+    L.oref_mtx_eval_local.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int]
+    mat = np.array(W.layered_material().as_list(), np.float32)
+    w = W.layered_closure_inputs(32768, seed=4)
+    n = w["N"].shape[1]
+    ev = np.zeros((4, n), np.float32)
+    smp = np.zeros((28, n), np.float32)
+    L.ocount_reset()
+    L.oref_mtx_layered(p(mat), p(w["N"]), p(w["V"]), p(w["L"]), p(w["params"]), p(w["rnd"]),
+                       p(ev), p(smp), n, 1)
+    res["closure"] = {"ops_per_point": L.ocount_get() / n, "points_counted": n,
+                      "sample": "first 32768 points of the closure workload (seed 4); "
+                                "1 eval + 4 samples per point"}
+    rev, rsmp = O.mtx_layered(mat, w["N"], w["V"], w["L"], w["params"], w["rnd"])
+    assert np.array_equal(ev, rev) and np.array_equal(smp, rsmp), "counting build differs"
+    q = W.surrogate_queries(32768, seed=6)
+    out = np.zeros((4, q.shape[1]), np.float32)
+    L.ocount_reset()
+    L.oref_mtx_eval_local(p(mat), p(q), p(out), q.shape[1], 1)
+    res["closure_eval"] = {"ops_per_point": L.ocount_get() / q.shape[1],
+                           "points_counted": q.shape[1],
+                           "sample": "32768 shading-frame queries (seed 6); 1 eval per query"}
+    assert np.array_equal(out, O.mtx_eval_local(mat, q)), "counting build differs"
     res["_doc"] = ("algorithmic 32-bit lane-ops per point from the op-counting oracle "
                    "(oracle/count/count_ops.py)")
     dst = os.path.join(ROOT, "openshadinglanguage_amd", "roofline_ops.json")

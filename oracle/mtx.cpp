@@ -211,16 +211,19 @@ float get_cosine(int table, int i)
     const float x = float(i) * (1.0f / 15.0f);
     return table == kLutMiniGGX ? smax(SQR(x), 1e-6f) : smax(x, 1e-6f);
 }
+// Row indices are clamped into the table: the reference asserts
+// 0 <= roughness, fresnel_index <= 1, where the clamps change nothing.
+inline int clamp_index(int k, int hi) { return k < 0 ? 0 : (k > hi ? hi : k); }
 float interpolate_emiss(const float* E, int Nf, float roughness, float fresnel_index, int i)
 {
     float rf = roughness * (kNr - 1);
-    int ra   = static_cast<int>(rf);
+    int ra   = clamp_index(static_cast<int>(rf), kNr - 1);
     int rb   = ra + 1 < kNr - 1 ? ra + 1 : kNr - 1;
     rf -= ra;
     if (Nf == 1)
         return LERP(rf, E[ra * kNc + i], E[rb * kNc + i]);
     float ff = fresnel_index * (Nf - 1);
-    int fa   = static_cast<int>(ff);
+    int fa   = clamp_index(static_cast<int>(ff), Nf - 1);
     int fb   = fa + 1 < Nf - 1 ? fa + 1 : Nf - 1;
     ff -= fa;
     return LERP(ff,
