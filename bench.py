@@ -151,6 +151,106 @@ def bench_closure(steps, warmup, cpu, cpu_passes):
     return res
 
 
+SURROGATE_TRAIN = 1 << 24   # training samples (seed 5)
+SURROGATE_CALIB = 1 << 20   # calibration queries (seed 7)
+SURROGATE_QUERIES = 1 << 23  # benchmark queries (seed 6)
+MLP_FLOP_PER_QUERY = 2 * (14 * 64 + 3 * 64 * 64 + 64 * 4)
+MFMA_BF16_PEAK = 2.5e15
+
+
+def bench_surrogate(steps, warmup, train_steps, batch=16384, budget=0.01):
+    """Train the closure surrogate on GPU-labelled data, calibrate its
+    routing, then time exact and surrogate evaluation of 8M queries."""
+    import torch
+    from openshadinglanguage_amd import check, lib
+    from openshadinglanguage_amd import surrogate as S
+    from openshadinglanguage_amd import workloads as W
+    dev = torch.device("cuda:0")
+    m = W.layered_material()
+    p = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def exact(q):
+        out = torch.empty((4, q.shape[1]), device=dev)
+        check(lib().oslmi_layered_eval_local(ctypes.byref(m), q.shape[1], p(q), p(out), sp))
+        return out
+
+    t0 = time.perf_counter()
+    qtr = torch.from_numpy(S.upper_hemisphere_queries(SURROGATE_TRAIN, seed=5)).to(dev)
+    ttr = exact(qtr)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    s = S.Surrogate(max_batch=batch)
+    s.fit_output_normalization(ttr)
+    print("surrogate: %d labelled samples in %.1f s; training" % (SURROGATE_TRAIN, t_data),
+          file=sys.stderr, flush=True)
+    tr = s.train(qtr, ttr, steps=train_steps, batch=batch, lr=2e-3, log_every=0,
+                 progress=lambda k: print("surrogate: step %d" % k, file=sys.stderr, flush=True))
+    _, final_loss = s.grads(qtr[:, :batch].contiguous(), ttr[:, :batch].contiguous())
+    del qtr, ttr
+
+    # calibration on held-out queries: network error per routing cell
+    qc = torch.from_numpy(S.upper_hemisphere_queries(SURROGATE_CALIB, seed=7)).to(dev)
+    tc = exact(qc)
+    pc = torch.empty_like(tc)
+    s.infer(qc, pc)
+    torch.cuda.synchronize()
+    table, _ = S.calibrate(pc.cpu().numpy(), tc.cpu().numpy(), qc.cpu().numpy(), budget)
+    s.set_routing(table)
+    del qc, tc, pc
+
+    # benchmark queries from the closure workload's distribution
+    qn = W.surrogate_queries(SURROGATE_QUERIES, seed=6)
+    q = torch.from_numpy(qn).to(dev)
+    n = q.shape[1]
+    ex = torch.empty((4, n), device=dev)
+    out = torch.empty((4, n), device=dev)
+    fbi = torch.empty(n, dtype=torch.int32, device=dev)
+    fbn = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def step_exact():
+        check(lib().oslmi_layered_eval_local(ctypes.byref(m), n, p(q), p(ex), sp))
+
+    def step_sur():
+        s.infer(q, out, material=m, fb_index=fbi, fb_count=fbn, stream=stream)
+
+    ms_exact, wall_exact = _timed(step_exact, stream, steps, warmup)
+    ms_sur, wall_sur = _timed(step_sur, stream, steps, warmup)
+    # the network alone (no routing, no fallback) for the MFMA roofline
+    net = torch.empty_like(out)
+    ms_net, _ = _timed(lambda: s.infer(q, net, stream=stream), stream, steps, warmup)
+    torch.cuda.synchronize()
+    ew, ep = S.relative_errors(out.cpu().numpy(), ex.cpu().numpy())
+    answered = ~((qn[4] > 0) & (qn[2] <= 0))
+    nfb = int(fbn.item())
+    flop = MLP_FLOP_PER_QUERY * n / (ms_net / 1e3)
+    return {
+        "workload": "8M closure queries (seed 6): exact layered closure vs 14-4x64-4 SiLU MLP "
+                    "(BF16 MFMA) with calibrated exact fallback",
+        "queries": n,
+        "exact_evals_per_s": round(n / (ms_exact / 1e3), 1),
+        "surrogate_evals_per_s": round(n / (ms_sur / 1e3), 1),
+        "speedup": round(ms_exact / ms_sur, 3),
+        "exact_kernel_ms": round(ms_exact, 4), "surrogate_ms": round(ms_sur, 4),
+        "network_only_ms": round(ms_net, 4),
+        "fallback_fraction": round(nfb / n, 4),
+        "routed_cells": int(table.sum()),
+        "rel_err_weight": {"mean": float(ew[answered].mean()),
+                           "p99": float(np.quantile(ew[answered], 0.99))},
+        "rel_err_pdf": {"mean": float(ep[answered].mean()),
+                        "p99": float(np.quantile(ep[answered], 0.99))},
+        "error_over": "queries with L above the surface (the rest are exact zeros)",
+        "training": {"samples": SURROGATE_TRAIN, "steps": tr["steps"], "batch": batch,
+                     "seconds": round(tr["seconds"], 2), "final_loss": round(final_loss, 6),
+                     "data_seconds": round(t_data, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(flop / 1e12, 2),
+                     "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": round(flop / MFMA_BF16_PEAK, 4), "traffic": None,
+                     "flop_per_query": MLP_FLOP_PER_QUERY, "kernel_ms": round(ms_net, 4)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +263,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the layered-closure measurement")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate training + measurement")
+    ap.add_argument("--train-steps", type=int, default=12000)
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
@@ -280,6 +383,9 @@ def main():
     if not args.no_closure:
         line["closure"] = bench_closure(max(5, args.steps // 2), args.warmup, not args.no_cpu,
                                         args.cpu_passes)
+    if not args.no_surrogate:
+        line["surrogate"] = bench_surrogate(max(5, args.steps // 2), args.warmup,
+                                            args.train_steps)
     print(json.dumps(line), flush=True)
 
 

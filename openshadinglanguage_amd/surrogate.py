@@ -131,8 +131,9 @@ class Surrogate:
         self.set_output_normalization(y.mean(1).tolist(), y.std(1).tolist())
 
     def train(self, queries, targets, steps, batch=16384, lr=2e-3, lr_min=1e-5, seed=0,
-              log_every=0, stream=None):
-        """Minibatch Adam over device planes queries [14, n], targets [4, n]."""
+              log_every=0, stream=None, progress=None):
+        """Minibatch Adam over device planes queries [14, n], targets [4, n].
+        progress(k) (optional) is called every 2000 steps."""
         import torch
         n = queries.shape[1]
         gen = torch.Generator(device=queries.device)
@@ -154,6 +155,8 @@ class Surrogate:
             if log_every and (k % log_every == 0 or k == steps - 1):
                 _, loss = self.grads(qb, tb)
                 log.append((k, loss))
+            if progress and k % 2000 == 0:
+                progress(k)
         torch.cuda.synchronize()
         return {"steps": steps, "batch": batch, "seconds": time.perf_counter() - t0, "log": log}
 
