@@ -340,17 +340,25 @@ __host__ __device__ inline int hidden_k(int s, int g, int j)
     return 16 * (2 * s + j / 4) + 4 * g + (j % 4);
 }
 
+// SiLU for inference with the hardware exp2 and reciprocal (v_exp_f32,
+// v_rcp_f32, about 1 ulp each): z / (1 + e^-z) = z * rcp(1 + 2^(-z log2 e)).
+// The result is rounded to bf16 next, far coarser than these errors.
+__device__ __forceinline__ float silu_fast(float z)
+{
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+}
+
 __device__ __forceinline__ bf16x8 act_pair(const f32x4& lo, const f32x4& hi)
 {
     bf16x8 r;
-    r[0] = (bf16)silu(lo[0]);
-    r[1] = (bf16)silu(lo[1]);
-    r[2] = (bf16)silu(lo[2]);
-    r[3] = (bf16)silu(lo[3]);
-    r[4] = (bf16)silu(hi[0]);
-    r[5] = (bf16)silu(hi[1]);
-    r[6] = (bf16)silu(hi[2]);
-    r[7] = (bf16)silu(hi[3]);
+    r[0] = (bf16)silu_fast(lo[0]);
+    r[1] = (bf16)silu_fast(lo[1]);
+    r[2] = (bf16)silu_fast(lo[2]);
+    r[3] = (bf16)silu_fast(lo[3]);
+    r[4] = (bf16)silu_fast(hi[0]);
+    r[5] = (bf16)silu_fast(hi[1]);
+    r[6] = (bf16)silu_fast(hi[2]);
+    r[7] = (bf16)silu_fast(hi[3]);
     return r;
 }
 
@@ -367,13 +375,113 @@ __device__ __forceinline__ int route_cell(float lx, float lz, float vx, float vz
 }
 
 // mode 0: network for every in-domain query (no routing table);
-// mode 1: routing table applied, fallback indices appended to fb / *fbn.
+// Sorts the queries: V.z > 0 and L.z <= 0 -> zeros (exact); inside the
+// training domain and (mode 1) not in a routed cell -> network list;
+// otherwise -> fallback list (mode 1) or NaN outputs (mode 0).
+// class of query i: 0 exact zero, 1 network, 2 fallback (mode 1) or not
+// answered (mode 0)
+__device__ __forceinline__ int route_class(const Domain& dom, const unsigned char* cells, int mode,
+                                           const float* __restrict__ q, int64_t n, int64_t i)
+{
+    const float lz = q[2 * n + i], vz = q[4 * n + i];
+    if (vz > 0.0f && lz <= 0.0f)
+        return 0;
+    bool inside = vz > 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const float p = q[(5 + k) * n + i];
+        inside = inside && p >= dom.lo[k] && p <= dom.hi[k];
+    }
+    if (inside && mode == 1 && cells[route_cell(q[i], lz, q[3 * n + i], vz, q[6 * n + i])])
+        inside = false;
+    return inside ? 1 : 2;
+}
+
+constexpr int ROUTE_CHUNK = 4096;  // queries per workgroup (16 rounds of 256)
+
+// Order-preserving compaction: workgroup b owns queries [b*4096, (b+1)*4096).
+// Pass 1 classifies them (one read of the query planes; classes kept in
+// LDS) and counts network and fallback queries, one atomic per list
+// reserves the workgroup's slots, pass 2 writes the indices in query order
+// (wave ballots plus a prefix over the 4 waves of each 256-query round).
+__global__ __launch_bounds__(256) void k_route(Domain dom, const unsigned char* __restrict__ cells,
+                                               int mode, const float* __restrict__ q, int64_t n,
+                                               float* __restrict__ out, int* __restrict__ net,
+                                               int* __restrict__ netn, int* __restrict__ fb,
+                                               int* __restrict__ fbn)
+{
+    __shared__ int wcnt[2][4];
+    __shared__ int base[2];
+    __shared__ signed char cls[ROUTE_CHUNK];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t c0 = (int64_t)blockIdx.x * ROUTE_CHUNK;
+    int mine[2] = { 0, 0 };
+    for (int r = 0; r < ROUTE_CHUNK / 256; r++) {
+        const int64_t i = c0 + r * 256 + threadIdx.x;
+        int c = -1;
+        if (i < n) {
+            c = route_class(dom, cells, mode, q, n, i);
+            if (c == 1)
+                mine[0]++;
+            else if (c == 2 && mode == 1)
+                mine[1]++;
+            else if (c == 0)
+                out[i] = out[n + i] = out[2 * n + i] = out[3 * n + i] = 0.0f;
+            else if (c == 2 && mode == 0)
+                out[i] = out[n + i] = out[2 * n + i] = out[3 * n + i] = __builtin_nanf("");
+        }
+        cls[r * 256 + threadIdx.x] = (signed char)c;
+    }
+    // block totals
+    for (int l = 0; l < 2; l++) {
+        int v = mine[l];
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_down(v, off);
+        if (lane == 0)
+            wcnt[l][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int l = threadIdx.x;
+        const int tot = wcnt[l][0] + wcnt[l][1] + wcnt[l][2] + wcnt[l][3];
+        base[l]       = tot ? atomicAdd(l == 0 ? netn : fbn, tot) : 0;
+    }
+    __syncthreads();
+    int run[2] = { base[0], base[1] };
+    __syncthreads();
+    for (int r = 0; r < ROUTE_CHUNK / 256; r++) {
+        const int64_t i = c0 + r * 256 + threadIdx.x;
+        const int c     = cls[r * 256 + threadIdx.x];
+        unsigned long long bal[2] = { __ballot(c == 1), __ballot(c == 2 && mode == 1) };
+        if (lane == 0) {
+            wcnt[0][wv] = __popcll(bal[0]);
+            wcnt[1][wv] = __popcll(bal[1]);
+        }
+        __syncthreads();
+        for (int l = 0; l < 2; l++) {
+            int before = 0, total = 0;
+            for (int w = 0; w < 4; w++) {
+                before += w < wv ? wcnt[l][w] : 0;
+                total += wcnt[l][w];
+            }
+            const bool take = (l == 0) ? (c == 1) : (c == 2 && mode == 1);
+            if (take) {
+                const int pos = run[l] + before + __popcll(bal[l] & ((1ull << lane) - 1ull));
+                (l == 0 ? net : fb)[pos] = (int)i;
+            }
+            run[l] += total;
+        }
+        __syncthreads();
+    }
+}
+
+// The network over the queries listed in net[0 .. *netn).
 __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __restrict__ frag,
                                                const f32x4* __restrict__ biasp,
-                                               const unsigned char* __restrict__ cells, int mode,
+                                               const int* __restrict__ net,
+                                               const int* __restrict__ netn,
                                                const float* __restrict__ q, int64_t n,
-                                               float* __restrict__ out, int* __restrict__ fb,
-                                               int* __restrict__ fbn)
+                                               float* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
@@ -397,88 +505,137 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
     w5[1] = frag[29 * 64 + lane];
     c5    = biasp[16 * 64 + lane];
 
-    const int64_t ntiles = (n + 15) / 16;
+    const int64_t cnt    = *netn;
+    const int64_t npairs = (cnt + 31) / 32;
     const int64_t wave   = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
-        const int64_t qi = tile * 16 + r;
-        const bool valid = qi < n;
-        // layer-1 operand: this lane group's 8 inputs of query qi (normalised)
-        bf16x8 x;
+    // Two 16-query tiles per iteration (independent MFMA/SiLU chains that the
+    // scheduler interleaves; one wave per SIMD holds all the weights), and a
+    // software pipeline over iterations: the next pair's query indices and
+    // raw inputs are loaded while the current pair is computed.
+    // loads are unconditional (clamped in-bounds addresses, results selected
+    // afterwards) so that they issue back to back without branches
+    auto load_idx = [&](int64_t pair, int qi[2]) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int k = 8 * g + j;
-            float v = 0.0f;
-            if (k < NIN && valid)
-                v = (q[k * n + qi] - kc.nm.in_off[k]) * kc.nm.in_scale[k];
-            x[j] = (bf16)v;
+        for (int u = 0; u < 2; u++) {
+            const int64_t e = (2 * pair + u) * 16 + r;
+            const bool ok   = e < cnt;
+            const int v     = net[ok ? e : 0];
+            qi[u]           = v | (ok ? 0 : -1);  // -1 marks no query
         }
-        f32x4 a[4];
+    };
+    auto load_x = [&](const int qi[2], float xr[2][8]) {
 #pragma unroll
-        for (int t = 0; t < 4; t++)
-            a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x, c1[t], 0, 0, 0);
-        bf16x8 h0 = act_pair(a[0], a[1]), h1 = act_pair(a[2], a[3]);
+        for (int u = 0; u < 2; u++) {
+            const int64_t qs = qi[u] >= 0 ? qi[u] : 0;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], h0, c2[t], 0, 0, 0);
-            a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][1], h1, a[t], 0, 0, 0);
-        }
-        h0 = act_pair(a[0], a[1]);
-        h1 = act_pair(a[2], a[3]);
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], h0, c3[t], 0, 0, 0);
-            a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][1], h1, a[t], 0, 0, 0);
-        }
-        h0 = act_pair(a[0], a[1]);
-        h1 = act_pair(a[2], a[3]);
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][0], h0, c4[t], 0, 0, 0);
-            a[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][1], h1, a[t], 0, 0, 0);
-        }
-        h0 = act_pair(a[0], a[1]);
-        h1 = act_pair(a[2], a[3]);
-        f32x4 y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[0], h0, c5, 0, 0, 0);
-        y       = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[1], h1, y, 0, 0, 0);
-
-        // lanes of group 0 hold outputs 0..3 of query qi
-        bool fallback = false;
-        if (g == 0 && valid) {
-            const float lx = q[qi], lz = q[2 * n + qi];
-            const float vx = q[3 * n + qi], vz = q[4 * n + qi];
-            bool below = vz > 0.0f && lz <= 0.0f;
-            bool inside = vz > 0.0f;
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                const float p = q[(5 + k) * n + qi];
-                inside = inside && p >= kc.dom.lo[k] && p <= kc.dom.hi[k];
+            for (int j = 0; j < 8; j++) {
+                const int k         = 8 * g + j;
+                const int kc_       = k < NIN ? k : NIN - 1;
+                const uint32_t keep = (k < NIN && qi[u] >= 0) ? 0xffffffffu : 0u;
+                xr[u][j] = __uint_as_float(__float_as_uint(q[kc_ * n + qs]) & keep);
             }
-            if (below) {
-                out[qi] = out[n + qi] = out[2 * n + qi] = out[3 * n + qi] = 0.0f;
-            } else if (!inside || (mode == 1 && cells[route_cell(lx, lz, vx, vz, q[6 * n + qi])])) {
-                fallback = true;
-            } else {
+        }
+    };
+    // this lane group's normalisation constants (inputs 8g .. 8g+7)
+    float noff[8], nscl[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int k = 8 * g + j;
+        noff[j]     = k < NIN ? kc.nm.in_off[k < NIN ? k : 0] : 0.0f;
+        nscl[j]     = k < NIN ? kc.nm.in_scale[k < NIN ? k : 0] : 0.0f;
+    }
+    int qcur[2], qnext[2];
+    float xcur[2][8];
+    load_idx(wave, qcur);
+    load_x(qcur, xcur);
+    load_idx(wave + nwaves, qnext);
+    for (int64_t pair = wave; pair < npairs; pair += nwaves) {
+        float xnext[2][8];
+        int qnn[2];
+        load_x(qnext, xnext);
+        load_idx(pair + 2 * nwaves, qnn);
+        f32x4 y2[2];
+        bf16x8 x[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                // layer-1 operand: this lane group's 8 inputs (normalised)
+                const int k = 8 * g + j;
+                x[u][j] = (bf16)(k < NIN ? (xcur[u][j] - noff[j]) * nscl[j] : 0.0f);
+            }
+        f32x4 a[2][4];
+        bf16x8 h0[2], h1[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x[u], c1[t], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            h0[u] = act_pair(a[u][0], a[u][1]);
+            h1[u] = act_pair(a[u][2], a[u][3]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], h0[u], c2[t], 0, 0, 0);
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][1], h1[u], a[u][t], 0, 0, 0);
+            }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            h0[u] = act_pair(a[u][0], a[u][1]);
+            h1[u] = act_pair(a[u][2], a[u][3]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], h0[u], c3[t], 0, 0, 0);
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][1], h1[u], a[u][t], 0, 0, 0);
+            }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            h0[u] = act_pair(a[u][0], a[u][1]);
+            h1[u] = act_pair(a[u][2], a[u][3]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][0], h0[u], c4[t], 0, 0, 0);
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][1], h1[u], a[u][t], 0, 0, 0);
+            }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            h0[u] = act_pair(a[u][0], a[u][1]);
+            h1[u] = act_pair(a[u][2], a[u][3]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[0], h0[u], c5, 0, 0, 0);
+            y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[1], h1[u], y2[u], 0, 0, 0);
+        }
+
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            // lanes of group 0 hold outputs 0..3 of their query
+            const int qi = qcur[u];
+            if (g == 0 && qi >= 0) {
 #pragma unroll
                 for (int o = 0; o < 4; o++)
-                    out[o * n + qi] = expf(y[o] * kc.nm.out_sd[o] + kc.nm.out_mu[o]);
+                    out[o * n + qi] = expf(y2[u][o] * kc.nm.out_sd[o] + kc.nm.out_mu[o]);
             }
         }
-        if (mode == 1) {
-            const unsigned long long ballot = __ballot(fallback);
-            if (ballot) {
-                int base = 0;
-                if (lane == 0)
-                    base = atomicAdd(fbn, __popcll(ballot));
-                base = __shfl(base, 0);
-                if (fallback) {
-                    const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
-                    fb[base + rank] = (int)qi;
-                }
-            } 
-        } else if (fallback) {
-            // mode 0 marks queries the network does not answer with NaN
-            out[qi] = out[n + qi] = out[2 * n + qi] = out[3 * n + qi] = __builtin_nanf("");
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            qcur[u]  = qnext[u];
+            qnext[u] = qnn[u];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                xcur[u][j] = xnext[u][j];
         }
     }
 }
@@ -510,7 +667,9 @@ struct Context {
     bf16x8* frag = nullptr;
     f32x4* biasp = nullptr;
     unsigned char* cells = nullptr;
-    int* fbn = nullptr;
+    int* fbn = nullptr;     // network-list count
+    int* netlist = nullptr;  // network-list indices (grown on demand)
+    int64_t netcap = 0;
     bool packed = false;
     std::vector<float> host_params;
 };
@@ -561,7 +720,7 @@ void ctx_free(Context* c)
     void* ptrs[] = { c->P, c->Mm, c->Vv, c->G, c->Wb, c->WTb, c->X, c->XT, c->Y, c->Z[1], c->Z[2],
                      c->Z[3], c->Z[4], c->Z[5], c->A[1], c->A[2], c->A[3], c->A[4], c->AT[1],
                      c->AT[2], c->AT[3], c->AT[4], c->dZ, c->dZT, c->dA, c->wpart, c->bpart,
-                     c->lpart, c->frag, c->biasp, c->cells, c->fbn };
+                     c->lpart, c->frag, c->biasp, c->cells, c->fbn, c->netlist };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -852,10 +1011,27 @@ extern "C" int oslmi_surrogate_infer(oslmi_surrogate* s, const oslmi_layered_mat
     const int mode = m ? 1 : 0;
     if (mode == 1 && (!d_fallback_index || !d_fallback_count))
         return set_error(OSLMI_ERR_INVALID, "oslmi_surrogate_infer: fallback buffers needed");
+    if (s->c.netcap < n) {
+        if (s->c.netlist) {
+            if (int rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return rc;
+            (void)hipFree(s->c.netlist);
+            s->c.netlist = nullptr;
+        }
+        if (int rc = check_hip(hipMalloc((void**)&s->c.netlist, n * sizeof(int)),
+                               "hipMalloc(network list)"))
+            return rc;
+        s->c.netcap = n;
+    }
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, s->c.fbn);
     if (mode == 1)
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_fallback_count);
+    const int64_t rblocks = (n + ROUTE_CHUNK - 1) / ROUTE_CHUNK;
+    hipLaunchKernelGGL(k_route, dim3((unsigned)rblocks), dim3(256), 0, st, s->c.dom, s->c.cells,
+                       mode, d_query, n, d_out, s->c.netlist, s->c.fbn, d_fallback_index,
+                       d_fallback_count);
     hipLaunchKernelGGL(k_infer, dim3(infer_grid(n)), dim3(256), 0, st, kc, s->c.frag, s->c.biasp,
-                       s->c.cells, mode, d_query, n, d_out, d_fallback_index, d_fallback_count);
+                       s->c.netlist, s->c.fbn, d_query, n, d_out);
     if (int rc = check_launch("oslmi_surrogate_infer"))
         return rc;
     if (mode == 1)

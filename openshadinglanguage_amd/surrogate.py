@@ -123,12 +123,14 @@ class Surrogate:
         return out, float(loss.value)
 
     def fit_output_normalization(self, targets):
-        """Mean and std of the log targets (device tensor [4, n])."""
+        """Centres the log targets (device tensor [4, n]) on their mean and
+        keeps unit scale, so the squared loss weighs every output's relative
+        error alike."""
         import torch
         y = torch.empty_like(targets)
         y[:3] = torch.log(torch.clamp(targets[:3], min=1e-6))
         y[3] = torch.log(torch.clamp(targets[3], min=1e-12))
-        self.set_output_normalization(y.mean(1).tolist(), y.std(1).tolist())
+        self.set_output_normalization(y.mean(1).tolist(), [1.0] * NOUT)
 
     def train(self, queries, targets, steps, batch=16384, lr=2e-3, lr_min=1e-5, seed=0,
               log_every=0, stream=None, progress=None):

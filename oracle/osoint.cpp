@@ -1,0 +1,760 @@
+// CPU oracle: an OSO interpreter for shader groups, one shading point at a
+// time (the reference's per-group, per-point execution, without
+// derivatives). TEST INFRASTRUCTURE ONLY (see oracle_core.h).
+//
+// Reads OSO text (the format oslc writes: src/liboslexec/osogram.y,
+// osolex.l), assembles layers with parameter connections like
+// ShaderGroupBegin/Shader/ConnectShaders (src/include/OSL/oslexec.h:634-742)
+// and executes each layer's ops in order, with OSO's jump targets for
+// if/for/while (src/liboslexec/batched_llvm_gen.cpp: llvm_gen_if :1068,
+// llvm_gen_loop_op :3806). Op semantics follow the reference:
+//   arithmetic with float -> triple promotion, int division by zero -> 0
+//   (opmath, batched_llvm_gen.cpp), mix x*(1-a)+y*a (:1868-1960),
+//   smoothstep (dual.h:1314), clamp = max(min(x,hi),lo) (stdosl.h:140),
+//   noise names of llvm_gen_noise (:6096-6134) via the oracle's noise,
+//   simplex and Gabor restatements, hash (wide_ophash.cpp), blackbody /
+//   luminance / transformc (opcolor.cpp), transform by a matrix
+//   (Imath multVecMatrix / multDirMatrix), closures diffuse and
+//   microfacet ggx (testrender shading.cpp:185-296) flattened into a lobe
+//   list and evaluated by the oracle's CompositeBSDF (closure.cpp).
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "closure.h"
+#include "color.h"
+#include "gabor.h"
+#include "noise.h"
+
+namespace oref {
+namespace oso {
+
+enum Base { kInt, kFloat, kTriple, kMatrix, kString, kClosure };
+
+struct Sym {
+    std::string kind;  // param oparam local temp const global
+    Base base = kFloat;
+    std::string name;
+    std::vector<float> fdef;
+    std::vector<std::string> sdef;
+};
+struct Op {
+    std::string name;
+    std::vector<int> args;
+    int jump[4] = { -1, -1, -1, -1 };
+};
+struct Master {
+    std::string type, name;
+    std::vector<Sym> syms;
+    std::vector<Op> ops;
+    int find(const std::string& n) const
+    {
+        for (size_t i = 0; i < syms.size(); ++i)
+            if (syms[i].name == n)
+                return (int)i;
+        return -1;
+    }
+};
+
+// ---- OSO reader ----------------------------------------------------------
+struct Tok {
+    enum K { Ident, Int, Float, Str, Hint, Punct } k;
+    std::string s;
+};
+bool tokenize(const std::string& line, std::vector<Tok>& out)
+{
+    size_t i = 0, n = line.size();
+    while (i < n) {
+        char c = line[i];
+        if (c == ' ' || c == '\t' || c == '\r') {
+            ++i;
+        } else if (c == '#') {
+            break;
+        } else if (c == '"') {
+            size_t j = i + 1;
+            std::string s;
+            while (j < n && line[j] != '"') {
+                if (line[j] == '\\' && j + 1 < n)
+                    ++j;
+                s += line[j++];
+            }
+            out.push_back({ Tok::Str, s });
+            i = j + 1;
+        } else if (c == '%') {
+            // hint: %name or %name{...}
+            size_t j = i + 1;
+            int depth = 0;
+            while (j < n && (depth > 0 || (line[j] != ' ' && line[j] != '\t'))) {
+                if (line[j] == '{')
+                    ++depth;
+                else if (line[j] == '}')
+                    --depth;
+                else if (line[j] == '"') {
+                    ++j;
+                    while (j < n && line[j] != '"')
+                        ++j;
+                }
+                ++j;
+            }
+            out.push_back({ Tok::Hint, line.substr(i, j - i) });
+            i = j;
+        } else if (isdigit((unsigned char)c) || ((c == '-' || c == '+' || c == '.') && i + 1 < n
+                                                  && (isdigit((unsigned char)line[i + 1])
+                                                      || line[i + 1] == '.'))) {
+            size_t j = i + 1;
+            bool isf = c == '.';
+            while (j < n && (isalnum((unsigned char)line[j]) || line[j] == '.' || line[j] == '-'
+                             || line[j] == '+')) {
+                if (line[j] == '.' || line[j] == 'e' || line[j] == 'E')
+                    isf = true;
+                if ((line[j] == '-' || line[j] == '+') && !(line[j - 1] == 'e' || line[j - 1] == 'E'))
+                    break;
+                ++j;
+            }
+            out.push_back({ isf ? Tok::Float : Tok::Int, line.substr(i, j - i) });
+            i = j;
+        } else if (isalpha((unsigned char)c) || c == '_' || c == '$') {
+            size_t j = i + 1;
+            while (j < n && (isalnum((unsigned char)line[j]) || line[j] == '_' || line[j] == '$'
+                             || line[j] == '.'))
+                ++j;
+            out.push_back({ Tok::Ident, line.substr(i, j - i) });
+            i = j;
+        } else {
+            out.push_back({ Tok::Punct, std::string(1, c) });
+            ++i;
+        }
+    }
+    return true;
+}
+
+bool parse_type(const std::string& t, Base& b)
+{
+    if (t == "int") b = kInt;
+    else if (t == "float") b = kFloat;
+    else if (t == "color" || t == "point" || t == "vector" || t == "normal") b = kTriple;
+    else if (t == "matrix") b = kMatrix;
+    else if (t == "string") b = kString;
+    else return false;
+    return true;
+}
+
+bool parse(const std::string& text, Master& m, std::string& err)
+{
+    size_t pos = 0;
+    int stage  = 0;  // 0 version, 1 shader line, 2 symbols, 3 code
+    while (pos < text.size()) {
+        size_t e = text.find('\n', pos);
+        if (e == std::string::npos)
+            e = text.size();
+        std::string line = text.substr(pos, e - pos);
+        pos              = e + 1;
+        std::vector<Tok> t;
+        tokenize(line, t);
+        if (t.empty())
+            continue;
+        if (stage == 0) {
+            stage = 1;
+            continue;
+        }
+        if (stage == 1) {
+            m.type = t[0].s;
+            m.name = t.size() > 1 ? t[1].s : "";
+            stage  = 2;
+            continue;
+        }
+        if (t[0].k == Tok::Ident && t[0].s == "code") {
+            stage = 3;
+            continue;
+        }
+        if (stage == 2) {
+            Sym s;
+            s.kind   = t[0].s;
+            size_t i = 1;
+            if (t[i].s == "closure") {
+                s.base = kClosure;
+                i += 2;
+            } else {
+                if (!parse_type(t[i].s, s.base)) {
+                    err = "bad type " + t[i].s;
+                    return false;
+                }
+                ++i;
+            }
+            s.name = t[i++].s;
+            for (; i < t.size(); ++i) {
+                if (t[i].k == Tok::Int || t[i].k == Tok::Float)
+                    s.fdef.push_back((float)strtod(t[i].s.c_str(), nullptr));
+                else if (t[i].k == Tok::Str)
+                    s.sdef.push_back(t[i].s);
+            }
+            m.syms.push_back(s);
+            continue;
+        }
+        Op op;
+        size_t i = 0;
+        op.name  = t[i++].s;
+        int nj   = 0;
+        for (; i < t.size(); ++i) {
+            if (t[i].k == Tok::Hint)
+                continue;
+            if (t[i].k == Tok::Int) {
+                if (nj < 4)
+                    op.jump[nj++] = atoi(t[i].s.c_str());
+                continue;
+            }
+            int s = m.find(t[i].s);
+            if (s < 0) {
+                err = "unknown symbol " + t[i].s;
+                return false;
+            }
+            op.args.push_back(s);
+        }
+        m.ops.push_back(op);
+    }
+    return true;
+}
+
+// ---- values ------------------------------------------------------------------
+struct Val {
+    int i      = 0;
+    float f[16] = { 0 };  // float: f[0]; triple: f[0..2]; matrix: f[0..15]
+    Composite cl;         // closure: lobe list (weights, not yet prepared)
+};
+
+struct Layer {
+    const Master* m;
+    std::string name;
+};
+struct Conn {
+    int src_layer, src_sym, dst_layer, dst_sym;
+};
+
+struct Group {
+    std::vector<std::unique_ptr<Master>> masters;
+    std::vector<Layer> layers;
+    std::vector<Conn> conns;
+    ColorSys cs;
+};
+
+struct Exec {
+    const Group& G;
+    std::vector<std::vector<Val>> vals;  // [layer][sym]
+    V3 P, N;
+    bool has_ci = false;
+    Composite ci;
+
+    explicit Exec(const Group& g) : G(g) {}
+
+    float F(int L, int s) const
+    {
+        const Sym& y = G.layers[L].m->syms[s];
+        return y.base == kInt ? (float)vals[L][s].i : vals[L][s].f[0];
+    }
+    int I(int L, int s) const
+    {
+        const Sym& y = G.layers[L].m->syms[s];
+        return y.base == kInt ? vals[L][s].i : (int)vals[L][s].f[0];
+    }
+    V3 T(int L, int s) const
+    {
+        const Sym& y = G.layers[L].m->syms[s];
+        if (y.base == kTriple)
+            return V3(vals[L][s].f[0], vals[L][s].f[1], vals[L][s].f[2]);
+        float v = F(L, s);
+        return V3(v, v, v);
+    }
+    const std::string& Sv(int L, int s) const { return G.layers[L].m->syms[s].sdef.at(0); }
+    Base B(int L, int s) const { return G.layers[L].m->syms[s].base; }
+    void setF(int L, int s, float v)
+    {
+        if (B(L, s) == kInt)
+            vals[L][s].i = (int)v;
+        else if (B(L, s) == kTriple)
+            vals[L][s].f[0] = vals[L][s].f[1] = vals[L][s].f[2] = v;
+        else
+            vals[L][s].f[0] = v;
+    }
+    void setT(int L, int s, V3 v)
+    {
+        vals[L][s].f[0] = v.x;
+        vals[L][s].f[1] = v.y;
+        vals[L][s].f[2] = v.z;
+    }
+
+    void init_layer(int L)
+    {
+        const Master& m = *G.layers[L].m;
+        vals[L].assign(m.syms.size(), Val());
+        for (size_t s = 0; s < m.syms.size(); ++s) {
+            const Sym& y = m.syms[s];
+            Val& v       = vals[L][s];
+            if (y.kind == "global") {
+                if (y.name == "P")
+                    setT(L, (int)s, P);
+                else if (y.name == "N" || y.name == "Ng")
+                    setT(L, (int)s, N);
+                continue;
+            }
+            if (y.base == kInt && !y.fdef.empty())
+                v.i = (int)y.fdef[0];
+            else if (y.base == kTriple && y.fdef.size() == 1)
+                v.f[0] = v.f[1] = v.f[2] = y.fdef[0];
+            else
+                for (size_t k = 0; k < y.fdef.size() && k < 16; ++k)
+                    v.f[k] = y.fdef[k];
+        }
+        for (const Conn& c : G.conns)
+            if (c.dst_layer == L) {
+                const Val& src = vals[c.src_layer][c.src_sym];
+                Base sb = B(c.src_layer, c.src_sym), db = B(L, c.dst_sym);
+                if (sb == db)
+                    vals[L][c.dst_sym] = src;
+                else if (db == kTriple)
+                    setT(L, c.dst_sym, T(c.src_layer, c.src_sym));
+                else
+                    setF(L, c.dst_sym, F(c.src_layer, c.src_sym));
+            }
+    }
+
+    void noise(int L, const Op& op)
+    {
+        const Master& m = *G.layers[L].m;
+        std::string name = op.name == "snoise"      ? "perlin"
+                           : op.name == "cellnoise" ? "cell"
+                           : op.name == "hashnoise" ? "hash"
+                                                    : "uperlin";
+        size_t first = 1;
+        if (op.args.size() > 1 && m.syms[op.args[1]].base == kString) {
+            name  = Sv(L, op.args[1]);
+            first = 2;
+        }
+        float c[4];
+        int dim = 0;
+        size_t a = first;
+        for (; a < op.args.size() && B(L, op.args[a]) != kString; ++a) {
+            if (B(L, op.args[a]) == kTriple) {
+                V3 v     = T(L, op.args[a]);
+                c[dim++] = v.x;
+                c[dim++] = v.y;
+                c[dim++] = v.z;
+            } else {
+                c[dim++] = F(L, op.args[a]);
+            }
+        }
+        const int r   = op.args[0];
+        const bool tr = B(L, r) == kTriple;
+        if (name == "perlin" || name == "snoise") {
+            if (tr) setT(L, r, snoise_v(dim, c));
+            else setF(L, r, snoise_f(dim, c));
+        } else if (name == "uperlin" || name == "noise") {
+            if (tr) setT(L, r, unoise_v(dim, c));
+            else setF(L, r, unoise_f(dim, c));
+        } else if (name == "cell" || name == "cellnoise") {
+            if (tr) setT(L, r, cellnoise_v(dim, c));
+            else setF(L, r, cellnoise_f(dim, c));
+        } else if (name == "hash" || name == "hashnoise") {
+            if (tr) setT(L, r, hashnoise_v(dim, c));
+            else setF(L, r, hashnoise_f(dim, c));
+        } else if (name == "simplex" || name == "usimplex") {
+            float s[3];
+            for (int k = 0; k < (tr ? 3 : 1); ++k) {
+                s[k] = simplex_f(dim, c, k, nullptr);
+                if (name == "usimplex")
+                    s[k] = 0.5f * (s[k] + 1.0f);
+            }
+            if (tr) setT(L, r, V3(s[0], s[1], s[2]));
+            else setF(L, r, s[0]);
+        } else if (name == "gabor") {
+            GaborOpts go;
+            for (; a + 1 < op.args.size(); a += 2) {
+                const std::string key = Sv(L, op.args[a]);
+                const float v         = F(L, op.args[a + 1]);
+                if (key == "anisotropic") go.anisotropic = (int)v;
+                else if (key == "do_filter") go.do_filter = (int)v;
+                else if (key == "bandwidth") go.bandwidth = v;
+                else if (key == "impulses") go.impulses = v;
+                else if (key == "direction") go.direction = T(L, op.args[a + 1]);
+            }
+            D2<float> p[3] = { D2<float>(c[0]), D2<float>(dim > 1 ? c[1] : 0.0f),
+                               D2<float>(dim > 2 ? c[2] : 0.0f) };
+            setF(L, r, gabor(dim, p, go).v);
+        } else {
+            fprintf(stderr, "osoint: noise '%s'\n", name.c_str());
+            abort();
+        }
+    }
+
+    void op1(int L, const Op& op)
+    {
+        const std::string& o = op.name;
+        auto a               = [&](int k) { return op.args[k]; };
+        if (o == "nop" || o == "useparam" || o == "end")
+            return;
+        const int r  = a(0);
+        const Base rb = B(L, r);
+        if (o == "assign") {
+            if (rb == kClosure) {
+                vals[L][r].cl = vals[L][a(1)].cl;
+                // Ci of the surface layer
+                if (G.layers[L].m->syms[r].name == "Ci" && G.layers[L].m->syms[r].kind == "global") {
+                    has_ci = true;
+                    ci     = vals[L][r].cl;
+                }
+            } else if (rb == kTriple) {
+                setT(L, r, T(L, a(1)));
+            } else if (rb == kInt) {
+                vals[L][r].i = I(L, a(1));
+            } else if (rb == kMatrix) {
+                vals[L][r] = vals[L][a(1)];
+            } else {
+                setF(L, r, F(L, a(1)));
+            }
+            return;
+        }
+        if (o == "add" || o == "sub" || o == "mul" || o == "div") {
+            if (rb == kClosure) {
+                if (o == "add") {
+                    Composite c = vals[L][a(1)].cl;
+                    const Composite& b = vals[L][a(2)].cl;
+                    for (int k = 0; k < b.n; ++k)
+                        c.add(b.lobes[k]);
+                    vals[L][r].cl = c;
+                } else {
+                    const int ci_ = B(L, a(1)) == kClosure ? a(1) : a(2);
+                    const int wi  = B(L, a(1)) == kClosure ? a(2) : a(1);
+                    Composite c   = vals[L][ci_].cl;
+                    const V3 w    = T(L, wi);
+                    for (int k = 0; k < c.n; ++k)
+                        c.lobes[k].weight = c.lobes[k].weight * w;
+                    vals[L][r].cl = c;
+                }
+                return;
+            }
+            if (rb == kInt) {
+                const int x = I(L, a(1)), y = I(L, a(2));
+                int v = o == "add" ? x + y : o == "sub" ? x - y : o == "mul" ? x * y
+                                                                 : (y == 0 ? 0 : x / y);
+                vals[L][r].i = v;
+                return;
+            }
+            if (rb == kTriple) {
+                const V3 x = T(L, a(1)), y = T(L, a(2));
+                V3 v = o == "add" ? x + y : o == "sub" ? x - y : o == "mul" ? x * y
+                                                                : V3(x.x / y.x, x.y / y.y, x.z / y.z);
+                setT(L, r, v);
+                return;
+            }
+            const float x = F(L, a(1)), y = F(L, a(2));
+            setF(L, r, o == "add" ? x + y : o == "sub" ? x - y : o == "mul" ? x * y : x / y);
+            return;
+        }
+        if (o == "neg") {
+            if (rb == kTriple) setT(L, r, -T(L, a(1)));
+            else if (rb == kInt) vals[L][r].i = -I(L, a(1));
+            else setF(L, r, -F(L, a(1)));
+            return;
+        }
+        if (o == "mix") {
+            if (rb == kTriple) {
+                const V3 x = T(L, a(1)), y = T(L, a(2)), t = T(L, a(3));
+                setT(L, r, V3(x.x * (1.0f - t.x) + y.x * t.x, x.y * (1.0f - t.y) + y.y * t.y,
+                              x.z * (1.0f - t.z) + y.z * t.z));
+            } else {
+                const float x = F(L, a(1)), y = F(L, a(2)), t = F(L, a(3));
+                setF(L, r, x * (1.0f - t) + y * t);
+            }
+            return;
+        }
+        if (o == "smoothstep") {
+            if (rb == kTriple) {
+                const V3 e0 = T(L, a(1)), e1 = T(L, a(2)), x = T(L, a(3));
+                setT(L, r, V3(smoothstep_f(e0.x, e1.x, x.x), smoothstep_f(e0.y, e1.y, x.y),
+                              smoothstep_f(e0.z, e1.z, x.z)));
+            } else {
+                setF(L, r, smoothstep_f(F(L, a(1)), F(L, a(2)), F(L, a(3))));
+            }
+            return;
+        }
+        if (o == "clamp") {
+            if (rb == kTriple) {
+                const V3 x = T(L, a(1)), lo = T(L, a(2)), hi = T(L, a(3));
+                setT(L, r, V3(fmaxf(fminf(x.x, hi.x), lo.x), fmaxf(fminf(x.y, hi.y), lo.y),
+                              fmaxf(fminf(x.z, hi.z), lo.z)));
+            } else if (rb == kInt) {
+                int x = I(L, a(1)), lo = I(L, a(2)), hi = I(L, a(3));
+                x     = x < hi ? x : hi;
+                vals[L][r].i = x > lo ? x : lo;
+            } else {
+                setF(L, r, fmaxf(fminf(F(L, a(1)), F(L, a(3))), F(L, a(2))));
+            }
+            return;
+        }
+        if (o == "floor" || o == "fabs" || o == "abs") {
+            auto f = [&](float x) { return o == "floor" ? floorf(x) : fabsf(x); };
+            if (rb == kTriple) {
+                const V3 x = T(L, a(1));
+                setT(L, r, V3(f(x.x), f(x.y), f(x.z)));
+            } else {
+                setF(L, r, f(F(L, a(1))));
+            }
+            return;
+        }
+        if (o == "compref") {
+            const V3 v  = T(L, a(1));
+            const int k = I(L, a(2));
+            setF(L, r, k == 0 ? v.x : (k == 1 ? v.y : v.z));
+            return;
+        }
+        if (o == "lt" || o == "gt" || o == "le" || o == "ge" || o == "eq" || o == "neq") {
+            const float x = F(L, a(1)), y = F(L, a(2));
+            int v = o == "lt" ? x < y : o == "gt" ? x > y : o == "le" ? x <= y : o == "ge" ? x >= y
+                                                                                 : o == "eq" ? x == y
+                                                                                             : x != y;
+            if (rb == kInt) vals[L][r].i = v;
+            else setF(L, r, (float)v);
+            return;
+        }
+        if (o == "noise" || o == "snoise" || o == "cellnoise" || o == "hashnoise") {
+            noise(L, op);
+            return;
+        }
+        if (o == "hash") {
+            float c[4];
+            int dim = 0;
+            for (size_t k = 1; k < op.args.size(); ++k) {
+                if (B(L, op.args[k]) == kTriple) {
+                    V3 v     = T(L, op.args[k]);
+                    c[dim++] = v.x;
+                    c[dim++] = v.y;
+                    c[dim++] = v.z;
+                } else if (B(L, op.args[k]) == kInt && op.args.size() == 2) {
+                    vals[L][r].i = hash_op_i(I(L, op.args[k]));
+                    return;
+                } else {
+                    c[dim++] = F(L, op.args[k]);
+                }
+            }
+            vals[L][r].i = hash_op_f(dim, c);
+            return;
+        }
+        if (o == "blackbody") {
+            setT(L, r, blackbody_rgb(G.cs, F(L, a(1))));
+            return;
+        }
+        if (o == "luminance") {
+            setF(L, r, luminance(G.cs, T(L, a(1))));
+            return;
+        }
+        if (o == "transformc") {
+            std::string from = "rgb", to;
+            int ci_ = 2;
+            if (op.args.size() == 4) {
+                from = Sv(L, a(1));
+                to   = Sv(L, a(2));
+                ci_  = 3;
+            } else {
+                to = Sv(L, a(1));
+            }
+            V3 c = T(L, a(ci_));
+            if (from == "XYZ" && (to == "rgb" || to == "RGB"))
+                c = xyz_to_rgb(G.cs, c);
+            setT(L, r, c);
+            return;
+        }
+        if (o == "matrix") {
+            for (int k = 0; k < 16; ++k)
+                vals[L][r].f[k] = F(L, a(k + 1));
+            return;
+        }
+        if (o == "transform") {
+            const float* m = vals[L][a(1)].f;
+            const V3 p     = T(L, a(2));
+            const Sym& ps  = G.layers[L].m->syms[a(2)];
+            (void)ps;
+            const float x = p.x * m[0] + p.y * m[4] + p.z * m[8] + m[12];
+            const float y = p.x * m[1] + p.y * m[5] + p.z * m[9] + m[13];
+            const float z = p.x * m[2] + p.y * m[6] + p.z * m[10] + m[14];
+            const float w = p.x * m[3] + p.y * m[7] + p.z * m[11] + m[15];
+            setT(L, r, V3(x / w, y / w, z / w));
+            return;
+        }
+        if (o == "closure") {
+            const std::string cname = Sv(L, a(1));
+            Composite c;
+            Lobe l;
+            l.weight = V3(1.0f);
+            if (cname == "diffuse") {
+                l.kind  = kLobeDiffuse;
+                l.N     = T(L, a(2));
+                l.alpha = 0.0f;
+                l.eta   = 0.0f;
+            } else {
+                l.kind  = kLobeGGXRefl;
+                l.N     = T(L, a(3));
+                l.alpha = F(L, a(5));
+                l.eta   = F(L, a(7));
+            }
+            c.add(l);
+            vals[L][r].cl = c;
+            return;
+        }
+        fprintf(stderr, "osoint: unsupported op %s\n", o.c_str());
+        abort();
+    }
+
+    // executes ops [b, e) of layer L with structured control flow
+    void block(int L, int b, int e)
+    {
+        const Master& m = *G.layers[L].m;
+        for (int i = b; i < e;) {
+            const Op& op = m.ops[i];
+            if (op.name == "if") {
+                if (I(L, op.args[0]))
+                    block(L, i + 1, op.jump[0]);
+                else
+                    block(L, op.jump[0], op.jump[1]);
+                i = op.jump[1];
+                continue;
+            }
+            if (op.name == "for" || op.name == "while") {
+                block(L, i + 1, op.jump[0]);
+                for (int guard = 0; guard < 1000000; ++guard) {
+                    block(L, op.jump[0], op.jump[1]);
+                    if (!I(L, op.args[0]))
+                        break;
+                    block(L, op.jump[1], op.jump[2]);
+                    block(L, op.jump[2], op.jump[3]);
+                }
+                i = op.jump[3];
+                continue;
+            }
+            op1(L, op);
+            ++i;
+        }
+    }
+
+    void run()
+    {
+        vals.assign(G.layers.size(), {});
+        has_ci = false;
+        for (size_t L = 0; L < G.layers.size(); ++L) {
+            init_layer((int)L);
+            block((int)L, 0, (int)G.layers[L].m->ops.size());
+        }
+    }
+};
+
+}  // namespace oso
+}  // namespace oref
+
+using namespace oref;
+using namespace oref::oso;
+
+extern "C" {
+
+void* oref_group_new(void)
+{
+    Group* g = new Group();
+    colorsys_init(g->cs, "Rec709");
+    return g;
+}
+void oref_group_free(void* h) { delete (Group*)h; }
+
+// Adds a layer from OSO text; returns 0 or -1 (error text to stderr).
+int oref_group_layer(void* h, const char* oso, const char* layername)
+{
+    Group* g = (Group*)h;
+    std::unique_ptr<Master> m(new Master());
+    std::string err;
+    if (!parse(oso, *m, err)) {
+        fprintf(stderr, "osoint: %s\n", err.c_str());
+        return -1;
+    }
+    g->layers.push_back({ m.get(), layername });
+    g->masters.push_back(std::move(m));
+    return 0;
+}
+
+int oref_group_connect(void* h, const char* srclayer, const char* srcparam, const char* dstlayer,
+                       const char* dstparam)
+{
+    Group* g = (Group*)h;
+    Conn c { -1, -1, -1, -1 };
+    for (size_t L = 0; L < g->layers.size(); ++L) {
+        if (g->layers[L].name == srclayer) {
+            c.src_layer = (int)L;
+            c.src_sym   = g->layers[L].m->find(srcparam);
+        }
+        if (g->layers[L].name == dstlayer) {
+            c.dst_layer = (int)L;
+            c.dst_sym   = g->layers[L].m->find(dstparam);
+        }
+    }
+    if (c.src_layer < 0 || c.dst_layer < 0 || c.src_sym < 0 || c.dst_sym < 0)
+        return -1;
+    g->conns.push_back(c);
+    return 0;
+}
+
+// Runs the group over n points. Globals P, N (3 planes each); outputs of
+// the last layer: fout (1 plane) and cout (3 planes) into out4; with wo/wi
+// non-null, the closure Ci evaluated at (wo, wi) into eval4 (R, G, B, pdf).
+void oref_group_run(void* h, const float* P, const float* N, const float* wo, const float* wi,
+                    int64_t n, float* out4, float* eval4, int nthreads)
+{
+    Group* g = (Group*)h;
+    const int last = (int)g->layers.size() - 1;
+    const int sf   = g->layers[last].m->find("fout");
+    const int sc   = g->layers[last].m->find("cout");
+    auto work = [&](int64_t b, int64_t e) {
+        Exec ex(*g);
+        for (int64_t i = b; i < e; ++i) {
+            ex.P = V3(P[i], P[n + i], P[2 * n + i]);
+            ex.N = V3(N[i], N[n + i], N[2 * n + i]);
+            ex.run();
+            out4[i]         = ex.F(last, sf);
+            const V3 c      = ex.T(last, sc);
+            out4[n + i]     = c.x;
+            out4[2 * n + i] = c.y;
+            out4[3 * n + i] = c.z;
+            if (eval4) {
+                Composite ci = ex.has_ci ? ex.ci : Composite();
+                const V3 o(wo[i], wo[n + i], wo[2 * n + i]);
+                const V3 w(wi[i], wi[n + i], wi[2 * n + i]);
+                ci.prepare(o);
+                BsdfSample s     = ci.eval(o, w);
+                eval4[i]         = s.weight.x;
+                eval4[n + i]     = s.weight.y;
+                eval4[2 * n + i] = s.weight.z;
+                eval4[3 * n + i] = s.pdf;
+            }
+        }
+    };
+    if (nthreads <= 1 || n < 256) {
+        work(0, n);
+        return;
+    }
+    std::atomic<int64_t> next(0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t)
+        th.emplace_back([&]() {
+            for (;;) {
+                int64_t b = next.fetch_add(256);
+                if (b >= n)
+                    break;
+                work(b, b + 256 < n ? b + 256 : n);
+            }
+        });
+    for (auto& t : th)
+        t.join();
+}
+
+}  // extern "C"

@@ -212,3 +212,41 @@ def mtx_weights(mat, N, V, params):
     p3 = np.zeros((3, n), np.float32)
     Lb.oref_mtx_weights(_p(m), *[_p(x) for x in a], _p(w9), _p(p3), n)
     return w9, p3
+
+
+# ---- OSO group interpreter (oracle/osoint.cpp) ---------------------------------
+class Group:
+    """A shader group run by the oracle's OSO interpreter."""
+
+    def __init__(self, layers, conns):
+        L = lib()
+        P, C = ctypes.c_void_p, ctypes.c_char_p
+        L.oref_group_new.restype = P
+        L.oref_group_free.argtypes = [P]
+        L.oref_group_layer.argtypes = [P, C, C]
+        L.oref_group_connect.argtypes = [P, C, C, C, C]
+        L.oref_group_run.argtypes = [P, P, P, P, P, ctypes.c_int64, P, P, ctypes.c_int]
+        self.L = L
+        self.h = L.oref_group_new()
+        for shadername, layername, text in layers:
+            assert L.oref_group_layer(self.h, text.encode(), layername.encode()) == 0
+        for s, sp, d, dp in conns:
+            assert L.oref_group_connect(self.h, s.encode(), sp.encode(), d.encode(),
+                                        dp.encode()) == 0
+
+    def __del__(self):
+        try:
+            self.L.oref_group_free(self.h)
+        except Exception:
+            pass
+
+    def run(self, P, N, wo=None, wi=None, nthreads=1):
+        a = [np.ascontiguousarray(x, np.float32) for x in (P, N)]
+        n = a[0].shape[1]
+        out = np.zeros((4, n), np.float32)
+        ev = np.zeros((4, n), np.float32) if wo is not None else None
+        w = [np.ascontiguousarray(x, np.float32) for x in (wo, wi)] if wo is not None else None
+        self.L.oref_group_run(self.h, _p(a[0]), _p(a[1]), _p(w[0]) if w else None,
+                              _p(w[1]) if w else None, n, _p(out), _p(ev) if w else None,
+                              nthreads)
+        return out, ev
