@@ -118,6 +118,62 @@ int oslmi_execute_batch(oslmi_shadingsys* ss, oslmi_group* g, const oslmi_batche
                         float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
                         void* stream);
 
+/* ---- Multi-group batches -----------------------------------------------
+ * Many groups shaded in one persistent launch. The reference executes one
+ * group per BatchedExecutor<W>::execute call (oslexec.h:1000-1004); a
+ * renderer with many materials loops over them (testshade.cpp:1886-1909
+ * does one group). A batch replaces that loop with one kernel: group k's
+ * points become tiles sorted by group, and every workgroup pulls tiles from
+ * a device counter until the list is empty. Item k of a bind is group k and
+ * takes the arguments oslmi_execute_batch takes. */
+typedef struct oslmi_batch oslmi_batch;
+typedef struct oslmi_batch_item {
+    oslmi_batched_globals globals;
+    float* d_outputs;
+    const float* d_wo;
+    const float* d_wi;
+    float* d_eval;
+} oslmi_batch_item;
+/* Launch shape: block (64, 128 or 256 work-items), points per work-item
+ * per tile (1..8; a tile is block * ppt points), workgroups per CU of the
+ * persistent grid (1..32), blackbody table in LDS (1) or read through the
+ * caches (0), tile order (0: groups in item order, 1: groups by measured
+ * cost per point, most expensive first; both keep each group's tiles
+ * together). */
+typedef struct oslmi_batch_config {
+    int block, ppt, blocks_per_cu, bb_lds, order;
+} oslmi_batch_config;
+/* Emits and compiles (hiprtc, code-object cache) the batch's persistent
+ * kernels over ngroups ended groups: groups_per_kernel groups per kernel
+ * (0: 8), the kernels launched one after another. load = 0 only fills the
+ * code-object cache (no device needed). */
+oslmi_batch* oslmi_batch_create(oslmi_shadingsys* ss, int ngroups, oslmi_group* const* groups,
+                                int groups_per_kernel, int load);
+void oslmi_batch_destroy(oslmi_batch* b);
+/* Longest compile time among the batch's kernels (compiled concurrently). */
+double oslmi_batch_jit_seconds(oslmi_batch* b);
+/* Number of persistent kernels a launch runs. */
+int oslmi_batch_kernels(oslmi_batch* b);
+/* Binds one item per group (nitems == ngroups) and uploads the argument
+ * blocks and tile list. Drops a captured graph. */
+int oslmi_batch_bind(oslmi_batch* b, int nitems, const oslmi_batch_item* items);
+int oslmi_batch_set_config(oslmi_batch* b, const oslmi_batch_config* cfg);
+int oslmi_batch_get_config(oslmi_batch* b, oslmi_batch_config* cfg);
+/* One persistent launch over every bound item (the captured graph when
+ * oslmi_batch_capture has run on this stream). */
+int oslmi_batch_launch(oslmi_batch* b, void* stream);
+/* The same kernel launched once per item, item after item (the reference's
+ * one-execute-per-group loop), for comparison. */
+int oslmi_batch_launch_per_group(oslmi_batch* b, void* stream);
+/* Captures oslmi_batch_launch into a hipGraph for replay on `stream`. */
+int oslmi_batch_capture(oslmi_batch* b, void* stream);
+/* Times `reps` launches of every candidate shape (block x ppt x
+ * blocks_per_cu x bb_lds x order) and keeps the fastest; writes it to
+ * *best and its mean time in ms to *best_ms. Runs the kernel, so the
+ * bound outputs are overwritten with the (identical) results. */
+int oslmi_batch_autotune(oslmi_batch* b, void* stream, int reps, oslmi_batch_config* best,
+                         double* best_ms);
+
 /* ---- Prebuilt group lowerings -----------------------------------------
  * The two benchmark groups compiled ahead of time (OSL sources in
  * openshadinglanguage_amd/shaders/). Each replaces one

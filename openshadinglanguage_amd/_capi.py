@@ -43,6 +43,17 @@ def _declare(L):
         "oslmi_group_output_planes": ([P, P], I),
         "oslmi_group_jit_seconds": ([P], ctypes.c_double),
         "oslmi_execute_batch": ([P, P, P, P, P, P, P, P], I),
+        "oslmi_batch_create": ([P, I, P, I, I], P),
+        "oslmi_batch_destroy": ([P], None),
+        "oslmi_batch_jit_seconds": ([P], ctypes.c_double),
+        "oslmi_batch_kernels": ([P], I),
+        "oslmi_batch_bind": ([P, I, P], I),
+        "oslmi_batch_set_config": ([P, P], I),
+        "oslmi_batch_get_config": ([P, P], I),
+        "oslmi_batch_launch": ([P, P], I),
+        "oslmi_batch_launch_per_group": ([P, P], I),
+        "oslmi_batch_capture": ([P, P], I),
+        "oslmi_batch_autotune": ([P, P, I, P, P], I),
         "oslmi_layered_closure": ([P, I64, P, P, P, P, P, P, P, P], I),
         "oslmi_layered_eval_local": ([P, I64, P, P, P], I),
         "oslmi_bsdl_tables": ([P, I64], I),

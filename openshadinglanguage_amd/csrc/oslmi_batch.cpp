@@ -1,0 +1,586 @@
+// Multi-group batches: persistent launches that shade every point of many
+// shader groups (host side of the emitted oslmi_batch_main kernels).
+//
+// The reference shades one group per BatchedExecutor<W>::execute call
+// (src/include/OSL/oslexec.h:1000-1004); a renderer with many materials
+// calls it group after group. On a 256-CU GPU that loop idles the machine
+// whenever a group has fewer points than it takes to fill it, and pays one
+// kernel's ramp-up and drain per group. A batch instead compiles the
+// groups' per-point code into a few persistent kernels ("parts", up to
+// groups_per_kernel groups each: one kernel over all groups would cost
+// minutes of compile time, and its register budget is the maximum over
+// every group, so small parts keep occupancy close to the groups' own).
+// Each group's points are cut into tiles of block * ppt points; a part's
+// tile list keeps each group's tiles together (group-sorted, so
+// neighbouring workgroups run the same code) and is ordered either as given
+// or by measured cost per tile, most expensive first (longest-processing-
+// time order, which shortens the tail). A persistent grid of blocks_per_cu
+// workgroups per CU pulls tiles from a device counter.
+//
+// The launch shape is tunable (block, ppt, blocks_per_cu, blackbody table
+// in LDS or not, tile order); oslmi_batch_autotune times every candidate on
+// the bound work and keeps the fastest. oslmi_batch_capture records the
+// launches into a hipGraph; every counter resets itself at the end of its
+// launch, so replays need no memset node.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "oslmi.h"
+#include "oslmi_color.h"
+#include "oslmi_group.h"
+#include "oslmi_internal.h"
+
+namespace oslmi {
+namespace {
+
+struct Part {
+    std::vector<int> items;  // batch items (= group indices) of this kernel
+    std::string source;
+    std::vector<char> code;
+    int rc = OSLMI_OK;
+    std::string err;
+    double seconds     = 0.0;
+    hipModule_t mod    = nullptr;
+    hipFunction_t fn   = nullptr;
+    int max_blocks[3]  = { 0, 0, 0 };  // resident workgroups per CU at block 64/128/256
+    int first_tile = 0, ntiles = 0;
+};
+
+int block_slot(int block) { return block == 64 ? 0 : block == 128 ? 1 : 2; }
+
+struct Batch {
+    ShadingSys* ss = nullptr;
+    std::vector<Group*> groups;
+    std::vector<Part> parts;
+    double jit_seconds = 0.0;
+    bool any_bb       = false;
+    int num_cu        = 0;
+    // bound work
+    bool bound = false;
+    std::vector<GroupArgs> hargs;
+    GroupArgs* dargs   = nullptr;
+    size_t dargs_cap   = 0;
+    std::vector<BatchTile> tiles;
+    std::vector<int> item_first, item_count;  // tile range of every item
+    BatchTile* dtiles  = nullptr;
+    size_t dtiles_cap  = 0;
+    int32_t* dcounter  = nullptr;  // one per item (per-group launches), one per part
+    size_t counter_cap = 0;
+    const float* bb    = nullptr;
+    std::vector<double> cost;  // measured ms per tile per item (order 1)
+    oslmi_batch_config cfg{ 256, 1, 8, 1, 0 };
+    // captured launches
+    hipGraph_t graph         = nullptr;
+    hipGraphExec_t gexec     = nullptr;
+    hipStream_t graph_stream = nullptr;
+
+    ~Batch()
+    {
+        drop_graph();
+        if (dargs)
+            (void)hipFree(dargs);
+        if (dtiles)
+            (void)hipFree(dtiles);
+        if (dcounter)
+            (void)hipFree(dcounter);
+        for (Part& p : parts)
+            if (p.mod)
+                (void)hipModuleUnload(p.mod);
+    }
+
+    void drop_graph()
+    {
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        gexec        = nullptr;
+        graph        = nullptr;
+        graph_stream = nullptr;
+    }
+
+    int load()
+    {
+        int dev = 0;
+        if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice"))
+            return rc;
+        if (int rc = check_hip(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev),
+                               "hipDeviceGetAttribute"))
+            return rc;
+        for (Part& p : parts) {
+            if (int rc = check_hip(hipModuleLoadData(&p.mod, p.code.data()), "hipModuleLoadData(batch)"))
+                return rc;
+            if (int rc = check_hip(hipModuleGetFunction(&p.fn, p.mod, "oslmi_batch_main"),
+                                   "hipModuleGetFunction(oslmi_batch_main)"))
+                return rc;
+            for (int block : { 64, 128, 256 }) {
+                int nb = 0;
+                if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p.fn, block, 0) != hipSuccess || nb < 1)
+                    nb = 1;
+                p.max_blocks[block_slot(block)] = nb;
+            }
+        }
+        return OSLMI_OK;
+    }
+
+    // Tile list for the current config: each item's points in tiles of
+    // block * ppt; within a part, items in the configured order.
+    int build_tiles()
+    {
+        const int64_t ts = (int64_t)cfg.block * cfg.ppt;
+        const int ni     = (int)hargs.size();
+        tiles.clear();
+        item_first.assign(ni, 0);
+        item_count.assign(ni, 0);
+        for (Part& p : parts) {
+            std::vector<int> order = p.items;
+            if (cfg.order == 1 && (int)cost.size() == ni)
+                std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+            p.first_tile = (int)tiles.size();
+            for (int k : order) {
+                item_first[k] = (int)tiles.size();
+                for (int64_t b = 0; b < hargs[k].n; b += ts) {
+                    BatchTile t;
+                    t.item  = k;
+                    t.pad   = 0;
+                    t.begin = b;
+                    t.end   = std::min<int64_t>(b + ts, hargs[k].n);
+                    tiles.push_back(t);
+                    if (tiles.size() > (size_t)0x7fffffff)
+                        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: too many tiles");
+                }
+                item_count[k] = (int)tiles.size() - item_first[k];
+            }
+            p.ntiles = (int)tiles.size() - p.first_tile;
+        }
+        if (tiles.size() > dtiles_cap) {
+            if (dtiles)
+                (void)hipFree(dtiles);
+            dtiles     = nullptr;
+            dtiles_cap = 0;
+            if (int rc = check_hip(hipMalloc((void**)&dtiles, tiles.size() * sizeof(BatchTile)),
+                                   "hipMalloc(batch tiles)"))
+                return rc;
+            dtiles_cap = tiles.size();
+        }
+        if (!tiles.empty())
+            if (int rc = check_hip(hipMemcpy(dtiles, tiles.data(), tiles.size() * sizeof(BatchTile),
+                                             hipMemcpyHostToDevice),
+                                   "hipMemcpy(batch tiles)"))
+                return rc;
+        return OSLMI_OK;
+    }
+
+    int part_of(int item) const
+    {
+        for (size_t p = 0; p < parts.size(); ++p)
+            for (int k : parts[p].items)
+                if (k == item)
+                    return (int)p;
+        return -1;
+    }
+
+    // One launch of part p's kernel over tiles [first, first + count) with
+    // counter slot c.
+    int launch_range(const Part& p, int first, int count, int c, hipStream_t stream)
+    {
+        if (count <= 0)
+            return OSLMI_OK;
+        int64_t grid = (int64_t)std::min(cfg.blocks_per_cu, p.max_blocks[block_slot(cfg.block)]) * num_cu;
+        grid         = std::max<int64_t>(1, std::min<int64_t>(grid, count));
+        BatchArgs B;
+        memset(&B, 0, sizeof(B));
+        B.args       = dargs;
+        B.tiles      = dtiles;
+        B.tile_begin = first;
+        B.tile_end   = first + count;
+        B.counter    = dcounter + c;
+        B.ppt        = cfg.ppt;
+        B.bb_lds     = cfg.bb_lds;
+        B.bb         = bb;
+        void* args[] = { &B };
+        return check_hip(hipModuleLaunchKernel(p.fn, (unsigned)grid, 1, 1, (unsigned)cfg.block, 1, 1, 0,
+                                               stream, args, nullptr),
+                         "hipModuleLaunchKernel(oslmi_batch_main)");
+    }
+
+    int launch_all(hipStream_t stream)
+    {
+        const int ni = (int)hargs.size();
+        for (size_t q = 0; q < parts.size(); ++q)
+            if (int rc = launch_range(parts[q], parts[q].first_tile, parts[q].ntiles, ni + (int)q, stream))
+                return rc;
+        return OSLMI_OK;
+    }
+
+    int launch_item(int k, hipStream_t stream)
+    {
+        return launch_range(parts[part_of(k)], item_first[k], item_count[k], k, stream);
+    }
+
+    int launch_per_group(hipStream_t stream)
+    {
+        for (int k = 0; k < (int)hargs.size(); ++k)
+            if (int rc = launch_item(k, stream))
+                return rc;
+        return OSLMI_OK;
+    }
+};
+
+int validate(const oslmi_batch_config& c)
+{
+    if (c.block != 64 && c.block != 128 && c.block != 256)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: block must be 64, 128 or 256");
+    if (c.ppt < 1 || c.ppt > 8)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: ppt must be in 1..8");
+    if (c.blocks_per_cu < 1 || c.blocks_per_cu > 32)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: blocks_per_cu must be in 1..32");
+    if (c.bb_lds != 0 && c.bb_lds != 1)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: bb_lds must be 0 or 1");
+    if (c.order != 0 && c.order != 1)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: order must be 0 or 1");
+    return OSLMI_OK;
+}
+
+// mean ms of `reps` calls of f on stream (after one untimed call)
+template <class F>
+int time_ms(hipStream_t stream, int reps, F f, double& ms)
+{
+    hipEvent_t e0, e1;
+    if (int rc = check_hip(hipEventCreate(&e0), "hipEventCreate"))
+        return rc;
+    if (int rc = check_hip(hipEventCreate(&e1), "hipEventCreate")) {
+        (void)hipEventDestroy(e0);
+        return rc;
+    }
+    int rc = f();
+    if (!rc)
+        rc = check_hip(hipEventRecord(e0, stream), "hipEventRecord");
+    for (int r = 0; r < reps && !rc; ++r)
+        rc = f();
+    if (!rc)
+        rc = check_hip(hipEventRecord(e1, stream), "hipEventRecord");
+    if (!rc)
+        rc = check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+    float t = 0.f;
+    if (!rc)
+        rc = check_hip(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+    ms = (double)t / std::max(1, reps);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+}  // namespace
+}  // namespace oslmi
+
+using namespace oslmi;
+
+struct oslmi_batch {
+    Batch b;
+};
+
+extern "C" {
+
+oslmi_batch* oslmi_batch_create(oslmi_shadingsys* s, int ngroups, oslmi_group* const* groups,
+                                int groups_per_kernel, int load)
+{
+    if (!s || ngroups <= 0 || !groups) {
+        set_error(OSLMI_ERR_INVALID, "oslmi_batch_create: need a shading system and >= 1 group");
+        return nullptr;
+    }
+    if (groups_per_kernel < 0) {
+        set_error(OSLMI_ERR_INVALID, "oslmi_batch_create: groups_per_kernel < 0");
+        return nullptr;
+    }
+    const int per = groups_per_kernel ? groups_per_kernel : 8;
+    oslmi_batch* h = new oslmi_batch;
+    Batch& b       = h->b;
+    b.ss           = &s->ss;
+    for (int k = 0; k < ngroups; ++k) {
+        if (!groups[k] || !groups[k]->g->ended) {
+            set_error(OSLMI_ERR_INVALID, "oslmi_batch_create: group " + std::to_string(k) + " is not ended");
+            delete h;
+            return nullptr;
+        }
+        for (int j = 0; j < k; ++j)
+            if (groups[j]->g == groups[k]->g) {
+                set_error(OSLMI_ERR_INVALID, "oslmi_batch_create: group " + std::to_string(k) + " listed twice");
+                delete h;
+                return nullptr;
+            }
+        b.groups.push_back(groups[k]->g);
+    }
+    // emit (the emitter records each group's outputs, under its lock)
+    for (int first = 0; first < ngroups; first += per) {
+        Part p;
+        std::vector<Group*> gs;
+        for (int k = first; k < std::min(ngroups, first + per); ++k) {
+            p.items.push_back(k);
+            gs.push_back(b.groups[k]);
+        }
+        std::string err;
+        bool ok;
+        {
+            std::vector<std::unique_lock<std::mutex>> locks;
+            for (Group* g : gs)
+                locks.emplace_back(g->mu);
+            ok = emit_batch(s->ss, gs, p.items, p.source, err);
+        }
+        if (!ok) {
+            set_error(OSLMI_ERR_COMPILE, "oslmi_batch_create: " + err);
+            delete h;
+            return nullptr;
+        }
+        b.any_bb = b.any_bb || p.source.find("s_bbl") != std::string::npos;
+        b.parts.push_back(std::move(p));
+    }
+    // compile the parts side by side (distinct hiprtc programs)
+    {
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        size_t next = 0;
+        while (next < b.parts.size()) {
+            std::vector<std::thread> pool;
+            for (unsigned t = 0; t < hw && next < b.parts.size(); ++t, ++next) {
+                Part* p = &b.parts[next];
+                pool.emplace_back([p, s, next] {
+                    p->rc = build_code_object(s->ss, p->source, "oslmi_batch" + std::to_string(next), p->code,
+                                              &p->seconds);
+                    if (p->rc)
+                        p->err = oslmi_last_error();
+                });
+            }
+            for (std::thread& t : pool)
+                t.join();
+        }
+        for (Part& p : b.parts) {
+            b.jit_seconds = std::max(b.jit_seconds, p.seconds);
+            if (p.rc) {
+                set_error(p.rc, p.err);
+                delete h;
+                return nullptr;
+            }
+        }
+    }
+    if (load && b.load()) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void oslmi_batch_destroy(oslmi_batch* h) { delete h; }
+
+double oslmi_batch_jit_seconds(oslmi_batch* h) { return h ? h->b.jit_seconds : 0.0; }
+
+int oslmi_batch_kernels(oslmi_batch* h) { return h ? (int)h->b.parts.size() : 0; }
+
+int oslmi_batch_bind(oslmi_batch* h, int nitems, const oslmi_batch_item* items)
+{
+    if (!h || !items)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_bind: null argument");
+    Batch& b = h->b;
+    if (b.parts.empty() || !b.parts[0].fn)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_bind: batch was created without loading");
+    if (nitems != (int)b.groups.size())
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_bind: need one item per group");
+    std::vector<GroupArgs> args(nitems);
+    for (int k = 0; k < nitems; ++k) {
+        const oslmi_batch_item& it = items[k];
+        std::string who = "oslmi_batch_bind item " + std::to_string(k);
+        if (int rc = make_group_args(*b.groups[k], &it.globals, it.d_outputs, it.d_wo, it.d_wi,
+                                     it.d_eval, who.c_str(), args[k]))
+            return rc;
+        if (it.globals.n > ((int64_t)1 << 40))
+            return set_error(OSLMI_ERR_INVALID, who + ": n too large");
+    }
+    if (b.any_bb) {
+        const ColorSysDev* cs = nullptr;
+        if (int rc = device_colorsys(&cs))
+            return rc;
+        b.bb = (const float*)((const char*)cs + offsetof(ColorSysDev, bb));
+    }
+    // launches still in flight may read the buffers about to change
+    if (int rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))
+        return rc;
+    b.drop_graph();
+    b.bound = false;
+    if ((size_t)nitems > b.dargs_cap) {
+        if (b.dargs)
+            (void)hipFree(b.dargs);
+        b.dargs     = nullptr;
+        b.dargs_cap = 0;
+        if (int rc = check_hip(hipMalloc((void**)&b.dargs, nitems * sizeof(GroupArgs)), "hipMalloc(batch args)"))
+            return rc;
+        b.dargs_cap = nitems;
+    }
+    if (int rc = check_hip(hipMemcpy(b.dargs, args.data(), nitems * sizeof(GroupArgs), hipMemcpyHostToDevice),
+                           "hipMemcpy(batch args)"))
+        return rc;
+    const size_t ncount = (size_t)nitems + b.parts.size();
+    if (ncount > b.counter_cap) {
+        if (b.dcounter)
+            (void)hipFree(b.dcounter);
+        b.dcounter    = nullptr;
+        b.counter_cap = 0;
+        if (int rc = check_hip(hipMalloc((void**)&b.dcounter, ncount * sizeof(int32_t)), "hipMalloc(batch counters)"))
+            return rc;
+        b.counter_cap = ncount;
+    }
+    if (int rc = check_hip(hipMemset(b.dcounter, 0, ncount * sizeof(int32_t)), "hipMemset(batch counters)"))
+        return rc;
+    b.hargs = args;
+    b.cost.clear();
+    if (int rc = b.build_tiles())
+        return rc;
+    b.bound = true;
+    return OSLMI_OK;
+}
+
+int oslmi_batch_set_config(oslmi_batch* h, const oslmi_batch_config* cfg)
+{
+    if (!h || !cfg)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_set_config: null argument");
+    if (int rc = validate(*cfg))
+        return rc;
+    Batch& b = h->b;
+    if (!b.bound) {
+        b.cfg = *cfg;
+        return OSLMI_OK;
+    }
+    if (int rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))
+        return rc;
+    b.drop_graph();
+    b.cfg = *cfg;
+    return b.build_tiles();
+}
+
+int oslmi_batch_get_config(oslmi_batch* h, oslmi_batch_config* cfg)
+{
+    if (!h || !cfg)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_get_config: null argument");
+    *cfg = h->b.cfg;
+    return OSLMI_OK;
+}
+
+int oslmi_batch_launch(oslmi_batch* h, void* stream)
+{
+    if (!h || !h->b.bound)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch: batch not bound");
+    Batch& b = h->b;
+    if (b.gexec && b.graph_stream == (hipStream_t)stream)
+        return check_hip(hipGraphLaunch(b.gexec, (hipStream_t)stream), "hipGraphLaunch");
+    return b.launch_all((hipStream_t)stream);
+}
+
+int oslmi_batch_launch_per_group(oslmi_batch* h, void* stream)
+{
+    if (!h || !h->b.bound)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch_per_group: batch not bound");
+    return h->b.launch_per_group((hipStream_t)stream);
+}
+
+int oslmi_batch_capture(oslmi_batch* h, void* stream)
+{
+    if (!h || !h->b.bound)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_capture: batch not bound");
+    if (!stream)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_capture: capture needs a created stream, not the null stream");
+    Batch& b       = h->b;
+    hipStream_t st = (hipStream_t)stream;
+    b.drop_graph();
+    if (int rc = check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+        return rc;
+    int lrc      = b.launch_all(st);
+    hipGraph_t g = nullptr;
+    int erc      = check_hip(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
+    if (lrc || erc) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return lrc ? lrc : erc;
+    }
+    b.graph = g;
+    if (int rc = check_hip(hipGraphInstantiate(&b.gexec, g, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+        b.drop_graph();
+        return rc;
+    }
+    b.graph_stream = st;
+    return OSLMI_OK;
+}
+
+int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_config* best,
+                         double* best_ms)
+{
+    if (!h || !h->b.bound)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_autotune: batch not bound");
+    Batch& b       = h->b;
+    hipStream_t st = (hipStream_t)stream;
+    reps           = std::max(1, reps);
+    if (int rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))
+        return rc;
+    b.drop_graph();
+    // cost of every item alone (block 256, one point per work-item), as ms
+    // per tile with as many of its tiles resident as fit
+    {
+        b.cfg = oslmi_batch_config{ 256, 1, 32, 1, 0 };
+        if (int rc = b.build_tiles())
+            return rc;
+        const int ni = (int)b.hargs.size();
+        std::vector<double> cost(ni, 0.0);
+        for (int k = 0; k < ni; ++k) {
+            if (b.item_count[k] == 0)
+                continue;
+            const Part& p       = b.parts[b.part_of(k)];
+            const int64_t slots = (int64_t)p.max_blocks[2] * b.num_cu;
+            double ms           = 0.0;
+            if (int rc = time_ms(st, 1, [&] { return b.launch_item(k, st); }, ms))
+                return rc;
+            cost[k] = ms * (double)std::min<int64_t>(b.item_count[k], slots) / b.item_count[k];
+        }
+        b.cost = cost;
+    }
+    oslmi_batch_config win{ 256, 1, 8, 1, 0 };
+    double win_ms = 1e30;
+    for (int block : { 64, 128, 256 }) {
+        int maxb = 1;
+        for (const Part& p : b.parts)
+            maxb = std::max(maxb, p.max_blocks[block_slot(block)]);
+        maxb = std::min(32, maxb);
+        std::vector<int> bpcs;
+        for (int v = 1; v < maxb; v *= 2)
+            bpcs.push_back(v);
+        bpcs.push_back(maxb);
+        for (int ppt : { 1, 2, 4 })
+            for (int bpc : bpcs)
+                for (int lds = 0; lds <= (b.any_bb ? 1 : 0); ++lds)
+                    for (int order = 0; order <= 1; ++order) {
+                        b.cfg = oslmi_batch_config{ block, ppt, bpc, b.any_bb ? lds : 1, order };
+                        if (int rc = b.build_tiles())
+                            return rc;
+                        double ms = 0.0;
+                        if (int rc = time_ms(st, reps, [&] { return b.launch_all(st); }, ms))
+                            return rc;
+                        if (ms < win_ms) {
+                            win_ms = ms;
+                            win    = b.cfg;
+                        }
+                    }
+    }
+    b.cfg = win;
+    if (int rc = b.build_tiles())
+        return rc;
+    if (best)
+        *best = win;
+    if (best_ms)
+        *best_ms = win_ms;
+    return OSLMI_OK;
+}
+
+}  // extern "C"

@@ -58,12 +58,14 @@ struct Emitter {
     std::vector<std::vector<char>> deriv;  // [layer][sym]
     std::ostringstream os;
     int label_counter = 0;
+    std::string lprefix = "L";  // unique per group inside a batch kernel
     int cur_layer     = 0;
     std::set<int> globals;
     bool uses_cs = false, uses_bb = false;
 
     Emitter(const ShadingSys& s, Group& gr) : ss(s), g(gr) {}
 
+    std::string lab(int n) const { return lprefix + std::to_string(n); }
     const ShaderMaster& M(int L) const { return *g.layers[L].master; }
     const OSymbol& S(int L, int s) const { return M(L).syms[s]; }
 
@@ -890,7 +892,7 @@ struct Emitter {
                 if (op.name == "dowhile") {
                     line("for (;;) {");
                     block(L, j1, j2, ls, le, layer_end_label);  // body
-                    line("L" + std::to_string(ls) + ":;");
+                    line(lab(ls) + ":;");
                     block(L, j2, j3, ls, le, layer_end_label);  // step
                     block(L, j0, j1, ls, le, layer_end_label);  // cond
                     line("if (!(" + cond + ")) break;");
@@ -900,27 +902,27 @@ struct Emitter {
                     block(L, j0, j1, ls, le, layer_end_label);  // cond
                     line("if (!(" + cond + ")) break;");
                     block(L, j1, j2, ls, le, layer_end_label);  // body
-                    line("L" + std::to_string(ls) + ":;");
+                    line(lab(ls) + ":;");
                     block(L, j2, j3, ls, le, layer_end_label);  // step
                     line("}");
                 }
-                line("L" + std::to_string(le) + ":;");
+                line(lab(le) + ":;");
                 line("}");
                 i = j3;
                 continue;
             }
             if (op.name == "break") {
-                line("goto L" + std::to_string(loop_end_label) + ";");
+                line("goto " + lab(loop_end_label) + ";");
                 ++i;
                 continue;
             }
             if (op.name == "continue") {
-                line("goto L" + std::to_string(loop_step_label) + ";");
+                line("goto " + lab(loop_step_label) + ";");
                 ++i;
                 continue;
             }
             if (op.name == "exit" || op.name == "return") {
-                line("goto L" + std::to_string(layer_end_label) + ";");
+                line("goto " + lab(layer_end_label) + ";");
                 ++i;
                 continue;
             }
@@ -937,7 +939,9 @@ struct Emitter {
         }
     }
 
-    bool run(std::string& src)
+    // Per-point code of the group: reads A (GroupArgs), i (point index), CS
+    // (colour system) and s_bb (blackbody table) from the enclosing scope.
+    bool per_point(std::string& code)
     {
         // topological layer order from connections (the group order must
         // already respect it, as ShaderGroupEnd requires)
@@ -1004,7 +1008,7 @@ struct Emitter {
             declare_layer(L);
             int le = ++label_counter;
             block(L, M(L).main_begin, (int)M(L).ops.size(), -1, -1, le);
-            line("L" + std::to_string(le) + ":;");
+            line(lab(le) + ":;");
             // hand layer outputs to downstream layers
             for (const Connection& c : g.conns)
                 if (c.src_layer == L)
@@ -1059,24 +1063,6 @@ struct Emitter {
         }
 
         std::ostringstream k;
-        k << "// Emitted by the oslmi group emitter for group '" << g.name << "'.\n";
-        k << "#include \"oslmi_rt.h\"\n#include \"oslmi_groupargs.h\"\n";
-        k << "using namespace oslmi;\nusing namespace oslmi::rt;\n";
-        k << "__device__ __forceinline__ V3 ld3(const GroupArgs& A, int g, int64_t i) {\n"
-             "    const float* p = A.g[g];\n"
-             "    return p ? V3(p[i], p[A.n + i], p[2 * A.n + i]) : V3(A.gu[g][0], A.gu[g][1], A.gu[g][2]);\n}\n";
-        k << "__device__ __forceinline__ float ld1(const GroupArgs& A, int g, int64_t i) {\n"
-             "    const float* p = A.g[g];\n    return p ? p[i] : A.gu[g][0];\n}\n";
-        k << "extern \"C\" __global__ __launch_bounds__(256) void oslmi_group_main(GroupArgs A)\n{\n";
-        if (uses_cs)
-            k << "    const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
-        if (uses_bb) {
-            k << "    __shared__ float s_bb[OSLMI_BB_TABLE * 3];\n"
-                 "    for (int j = threadIdx.x; j < OSLMI_BB_TABLE * 3; j += blockDim.x) s_bb[j] = CS->bb[j];\n"
-                 "    __syncthreads();\n";
-        }
-        k << "    int64_t stride = (int64_t)gridDim.x * blockDim.x;\n";
-        k << "    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {\n";
         if (g.surface_closure)
             k << "        Closure ci_;\n";
         k << carriers.str();
@@ -1091,11 +1077,46 @@ struct Emitter {
                  "            A.eval[2 * A.n + i] = e_.weight.z; A.eval[3 * A.n + i] = e_.pdf;\n"
                  "        }\n";
         }
-        k << "    }\n}\n";
-        src = k.str();
+        code            = k.str();
         g.uses_colorsys = uses_cs;
         g.globals_used.assign(globals.begin(), globals.end());
         return true;
+    }
+
+    // standalone kernel of one group: a grid-stride loop over its points
+    bool run(std::string& src)
+    {
+        std::string code;
+        if (!per_point(code))
+            return false;
+        std::ostringstream k;
+        k << "// Emitted by the oslmi group emitter for group '" << g.name << "'.\n";
+        k << prelude();
+        k << "extern \"C\" __global__ __launch_bounds__(256) void oslmi_group_main(GroupArgs A)\n{\n";
+        if (uses_cs)
+            k << "    const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
+        if (uses_bb) {
+            k << "    __shared__ float s_bb[OSLMI_BB_TABLE * 3];\n"
+                 "    for (int j = threadIdx.x; j < OSLMI_BB_TABLE * 3; j += blockDim.x) s_bb[j] = CS->bb[j];\n"
+                 "    __syncthreads();\n";
+        }
+        k << "    int64_t stride = (int64_t)gridDim.x * blockDim.x;\n";
+        k << "    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {\n";
+        k << code;
+        k << "    }\n}\n";
+        src = k.str();
+        return true;
+    }
+
+    static std::string prelude()
+    {
+        return "#include \"oslmi_rt.h\"\n#include \"oslmi_groupargs.h\"\n"
+               "using namespace oslmi;\nusing namespace oslmi::rt;\n"
+               "__device__ __forceinline__ V3 ld3(const GroupArgs& A, int g, int64_t i) {\n"
+               "    const float* p = A.g[g];\n"
+               "    return p ? V3(p[i], p[A.n + i], p[2 * A.n + i]) : V3(A.gu[g][0], A.gu[g][1], A.gu[g][2]);\n}\n"
+               "__device__ __forceinline__ float ld1(const GroupArgs& A, int g, int64_t i) {\n"
+               "    const float* p = A.g[g];\n    return p ? p[i] : A.gu[g][0];\n}\n";
     }
 };
 
@@ -1108,6 +1129,89 @@ bool emit_group(const ShadingSys& ss, Group& g, std::string& src, std::string& e
         err = e.err.empty() ? "emit failed" : e.err;
         return false;
     }
+    return true;
+}
+
+// One persistent kernel for a list of groups. Work arrives as tiles
+// {item, begin, end} (BatchTile), sorted by group by the host; every
+// workgroup pulls the next tile from a device counter and runs the tile's
+// group body over blockDim.x * ppt consecutive points (ppt strided passes,
+// so every pass is a coalesced sweep of the SoA planes). The last
+// workgroup to find the list empty resets the counter, so repeated
+// launches (and graph replays) need no clearing.
+bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
+                const std::vector<int>& ids, std::string& src, std::string& err)
+{
+    if (ids.size() != groups.size()) {
+        err = "one item id per group";
+        return false;
+    }
+    std::ostringstream k;
+    k << "// Emitted by the oslmi group emitter: persistent batch kernel over " << groups.size()
+      << " groups.\n";
+    k << Emitter::prelude();
+    bool any_bb = false;
+    std::vector<std::string> cases;
+    for (size_t j = 0; j < groups.size(); ++j) {
+        const int gi = ids[j];
+        Emitter e(ss, *groups[j]);
+        e.lprefix = "G" + std::to_string(gi) + "_L";
+        std::string code;
+        if (!e.per_point(code)) {
+            err = "group " + groups[j]->name + ": " + (e.err.empty() ? "emit failed" : e.err);
+            return false;
+        }
+        any_bb = any_bb || e.uses_bb;
+        // a case keeps taking tiles while they belong to its group, so the
+        // group's loop invariants live only inside its case
+        std::ostringstream c;
+        c << "    case " << gi << ": {  // group '" << groups[j]->name << "'\n";
+        c << "        const GroupArgs& A = args[" << gi << "];\n";
+        if (e.uses_cs)
+            c << "        const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
+        c << "        for (;;) {\n";
+        c << "#pragma unroll 1\n";
+        c << "        for (int q_ = 0; q_ < B.ppt; ++q_) {\n";
+        c << "        int64_t i = T.begin + (int64_t)q_ * blockDim.x + threadIdx.x;\n";
+        c << "        if (i < T.end) {\n";
+        c << code;
+        c << "        }\n        }\n";
+        c << "        t = fetch_tile(B.counter, &s_tile);\n";
+        c << "        if (t >= ntiles) break;\n";
+        c << "        T = B.tiles[B.tile_begin + t];\n";
+        c << "        if (T.item != " << gi << ") break;\n";
+        c << "        }\n        break;\n    }\n";
+        cases.push_back(c.str());
+    }
+    k << "// next tile index for the whole workgroup (the barrier in front keeps\n"
+         "// the previous index readable until every work-item has taken it)\n"
+         "__device__ __forceinline__ int fetch_tile(int32_t* counter, int* s_tile) {\n"
+         "    __syncthreads();\n"
+         "    if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1);\n"
+         "    __syncthreads();\n"
+         "    return *s_tile;\n}\n";
+    k << "extern \"C\" __global__ __launch_bounds__(256) void oslmi_batch_main(BatchArgs B)\n{\n";
+    k << "    const GroupArgs* __restrict__ args = B.args;\n";
+    k << "    __shared__ int s_tile;\n";
+    if (any_bb) {
+        k << "    __shared__ float s_bbl[OSLMI_BB_TABLE * 3];\n"
+             "    const float* s_bb = B.bb;\n"
+             "    if (B.bb_lds) {\n"
+             "        for (int j = threadIdx.x; j < OSLMI_BB_TABLE * 3; j += blockDim.x) s_bbl[j] = B.bb[j];\n"
+             "        s_bb = s_bbl;\n"
+             "    }\n";
+    }
+    k << "    const int ntiles = B.tile_end - B.tile_begin;\n"
+         "    int t = fetch_tile(B.counter, &s_tile);\n"
+         "    while (t < ntiles) {\n"
+         "        BatchTile T = B.tiles[B.tile_begin + t];\n"
+         "        switch (T.item) {\n";
+    for (const std::string& c : cases)
+        k << c;
+    k << "        default: t = fetch_tile(B.counter, &s_tile); break;\n        }\n    }\n"
+         "    // one failed fetch per workgroup: the last one clears the counter\n"
+         "    if (threadIdx.x == 0 && t == ntiles + (int)gridDim.x - 1) atomicExch(B.counter, 0);\n}\n";
+    src = k.str();
     return true;
 }
 

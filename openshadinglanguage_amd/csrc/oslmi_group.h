@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "oslmi.h"
 #include "oslmi_groupargs.h"
 #include "oslmi_oso.h"
 
@@ -79,4 +80,31 @@ struct ShadingSys {
 // g.output_planes, g.surface_closure, g.uses_colorsys, g.globals_used.
 bool emit_group(const ShadingSys& ss, Group& g, std::string& src, std::string& err);
 
+// Emits the HIP source of the persistent batch kernel oslmi_batch_main
+// (BatchArgs) over the given groups; groups[j] is batch item ids[j].
+bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
+                const std::vector<int>& ids, std::string& src, std::string& err);
+
+// hiprtc compile of src for gfx950 through the shading system's code-object
+// cache (key: source, options and device library headers).
+int build_code_object(const ShadingSys& ss, const std::string& src, const std::string& name,
+                      std::vector<char>& code, double* seconds);
+
+// The colour system (matrices, blackbody table) in device memory.
+struct ColorSysDev;
+int device_colorsys(const ColorSysDev** out);
+
+// Kernel argument block of one execution of group g (validates the
+// arguments like BatchedExecutor::execute; `who` prefixes error messages).
+int make_group_args(const Group& g, const oslmi_batched_globals* bg, float* d_outputs,
+                    const float* d_wo, const float* d_wi, float* d_eval, const char* who,
+                    GroupArgs& A);
 }  // namespace oslmi
+
+// Opaque handles of the C ABI.
+struct oslmi_shadingsys {
+    oslmi::ShadingSys ss;
+};
+struct oslmi_group {
+    oslmi::Group* g;
+};

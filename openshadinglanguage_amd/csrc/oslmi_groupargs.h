@@ -30,4 +30,22 @@ struct GroupArgs {
     float* eval;
 };
 
+// One unit of work of the persistent batch kernel: points [begin, end) of
+// batch item `item` (which is also the group's case in the kernel).
+struct BatchTile {
+    int32_t item;
+    int32_t pad;
+    int64_t begin, end;
+};
+
+struct BatchArgs {
+    const GroupArgs* args;   // one per item, device memory
+    const BatchTile* tiles;  // device memory
+    int32_t tile_begin, tile_end;
+    int32_t* counter;        // tile fetch counter, zero between launches
+    int32_t ppt;             // passes of blockDim.x points per tile
+    int32_t bb_lds;          // stage the blackbody table in LDS
+    const float* bb;         // blackbody table (global memory)
+};
+
 }  // namespace oslmi

@@ -87,12 +87,11 @@ void mkdirs(const std::string& d)
 const char* kCompileOpts[] = { "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
                                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops" };
 
-int compile_group(ShadingSys& ss, Group& g, bool load = true)
+}  // namespace
+
+int build_code_object(const ShadingSys& ss, const std::string& src, const std::string& name,
+                      std::vector<char>& code, double* seconds)
 {
-    std::string src, err;
-    if (!emit_group(ss, g, src, err))
-        return set_error(OSLMI_ERR_COMPILE, "emit " + g.name + ": " + err);
-    g.source = src;
     std::string opts_key;
     for (const char* o : kCompileOpts)
         opts_key += std::string(o) + " ";
@@ -105,12 +104,11 @@ int compile_group(ShadingSys& ss, Group& g, bool load = true)
         hdrs.append(c.begin(), c.end());
     }
     std::string key = hex64(fnv1a(src + "|" + opts_key + "|" + hdrs));
-    std::vector<char> code;
     std::string cache_path = ss.cache_dir.empty() ? "" : ss.cache_dir + "/" + key + ".hsaco";
     auto t0 = std::chrono::steady_clock::now();
     if (cache_path.empty() || !read_file(cache_path, code)) {
         hiprtcProgram prog;
-        if (hiprtcCreateProgram(&prog, src.c_str(), (g.name + ".hip").c_str(), 0, nullptr, nullptr)
+        if (hiprtcCreateProgram(&prog, src.c_str(), (name + ".hip").c_str(), 0, nullptr, nullptr)
             != HIPRTC_SUCCESS)
             return set_error(OSLMI_ERR_COMPILE, "hiprtcCreateProgram failed");
         std::vector<const char*> opts(std::begin(kCompileOpts), std::end(kCompileOpts));
@@ -124,7 +122,7 @@ int compile_group(ShadingSys& ss, Group& g, bool load = true)
             hiprtcGetProgramLog(prog, &log[0]);
         if (rc != HIPRTC_SUCCESS) {
             hiprtcDestroyProgram(&prog);
-            return set_error(OSLMI_ERR_COMPILE, "hiprtc compile of group " + g.name + " failed:\n" + log);
+            return set_error(OSLMI_ERR_COMPILE, "hiprtc compile of " + name + " failed:\n" + log);
         }
         size_t sz = 0;
         hiprtcGetCodeSize(prog, &sz);
@@ -140,7 +138,55 @@ int compile_group(ShadingSys& ss, Group& g, bool load = true)
             rename(tmp.c_str(), cache_path.c_str());
         }
     }
-    g.jit_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return OSLMI_OK;
+}
+
+int make_group_args(const Group& g, const oslmi_batched_globals* bg, float* d_outputs,
+                    const float* d_wo, const float* d_wi, float* d_eval, const char* who,
+                    GroupArgs& A)
+{
+    std::string w(who);
+    if (bg->n < 0)
+        return set_error(OSLMI_ERR_INVALID, w + ": n < 0");
+    if (bg->n > 0 && g.output_planes > 0 && !d_outputs)
+        return set_error(OSLMI_ERR_INVALID, w + ": group has outputs but d_outputs is NULL");
+    if (d_eval && (!g.surface_closure || !d_wo || !d_wi))
+        return set_error(OSLMI_ERR_INVALID,
+                         w + ": closure evaluation needs a surface closure and wo/wi");
+    memset(&A, 0, sizeof(A));
+    A.n = bg->n;
+    for (int k = 0; k < kG_COUNT; ++k) {
+        A.g[k] = bg->planes[k];
+        for (int c = 0; c < 3; ++c)
+            A.gu[k][c] = bg->uniform[k][c];
+    }
+    for (int p = 0; p < g.output_planes; ++p)
+        A.out[p] = d_outputs + (int64_t)p * bg->n;
+    A.wo   = d_wo;
+    A.wi   = d_wi;
+    A.eval = d_eval;
+    if (g.uses_colorsys) {
+        const ColorSysDev* cs = nullptr;
+        if (int rc = device_colorsys(&cs))
+            return rc;
+        A.colorsys = cs;
+    }
+    return OSLMI_OK;
+}
+
+namespace {
+
+int compile_group(ShadingSys& ss, Group& g, bool load = true)
+{
+    std::string src, err;
+    if (!emit_group(ss, g, src, err))
+        return set_error(OSLMI_ERR_COMPILE, "emit " + g.name + ": " + err);
+    g.source = src;
+    std::vector<char> code;
+    if (int rc = build_code_object(ss, src, g.name, code, &g.jit_seconds))
+        return rc;
     if (!load)
         return OSLMI_OK;
     if (g.module) {
@@ -170,13 +216,6 @@ int precompile_group(ShadingSys& ss, Group& g)
 }  // namespace oslmi
 
 using namespace oslmi;
-
-struct oslmi_shadingsys {
-    ShadingSys ss;
-};
-struct oslmi_group {
-    Group* g;
-};
 
 extern "C" {
 
@@ -395,34 +434,11 @@ int oslmi_execute_batch(oslmi_shadingsys* s, oslmi_group* gh, const oslmi_batche
         if (int rc = oslmi_jit_group(s, gh))
             return rc;
     }
-    if (bg->n < 0)
-        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch: n < 0");
+    GroupArgs A;
+    if (int rc = make_group_args(g, bg, d_outputs, d_wo, d_wi, d_eval, "oslmi_execute_batch", A))
+        return rc;
     if (bg->n == 0)
         return OSLMI_OK;
-    if (g.output_planes > 0 && !d_outputs)
-        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch: group has outputs but d_outputs is NULL");
-    if (d_eval && (!g.surface_closure || !d_wo || !d_wi))
-        return set_error(OSLMI_ERR_INVALID,
-                         "oslmi_execute_batch: closure evaluation needs a surface closure and wo/wi");
-    GroupArgs A;
-    memset(&A, 0, sizeof(A));
-    A.n = bg->n;
-    for (int k = 0; k < kG_COUNT; ++k) {
-        A.g[k] = bg->planes[k];
-        for (int c = 0; c < 3; ++c)
-            A.gu[k][c] = bg->uniform[k][c];
-    }
-    for (int p = 0; p < g.output_planes; ++p)
-        A.out[p] = d_outputs + (int64_t)p * bg->n;
-    A.wo   = d_wo;
-    A.wi   = d_wi;
-    A.eval = d_eval;
-    if (g.uses_colorsys) {
-        const ColorSysDev* cs = nullptr;
-        if (int rc = device_colorsys(&cs))
-            return rc;
-        A.colorsys = cs;
-    }
     void* args[] = { &A };
     if (int rc = check_hip(hipModuleLaunchKernel(g.kernel, grid_for(bg->n), 1, 1, 256, 1, 1, 0,
                                                  (hipStream_t)stream, args, nullptr),

@@ -15,6 +15,7 @@
 #else
 // hiprtc provides the fixed-width integer types in an internal namespace
 typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::int32_t int32_t;
 typedef __hip_internal::int64_t int64_t;
 typedef __hip_internal::uint64_t uint64_t;
 #endif

@@ -27,8 +27,42 @@ class BatchedGlobals(ctypes.Structure):
                 ("uniform", (ctypes.c_float * 3) * 16)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("globals", BatchedGlobals),
+                ("d_outputs", ctypes.c_void_p),
+                ("d_wo", ctypes.c_void_p),
+                ("d_wi", ctypes.c_void_p),
+                ("d_eval", ctypes.c_void_p)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int), ("ppt", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("bb_lds", ctypes.c_int), ("order", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def batched_globals(n, globals_):
+    """oslmi_batched_globals for n points. globals_: dict name -> CUDA
+    tensor of planes (varying) or a float / tuple of floats (uniform)."""
+    bg = BatchedGlobals()
+    bg.n = n
+    for k, name in enumerate(GLOBALS):
+        v = globals_.get(name)
+        if v is None:
+            continue
+        if hasattr(v, "data_ptr"):
+            bg.planes[k] = v.data_ptr()
+        else:
+            vals = list(v) if hasattr(v, "__len__") else [v]
+            for c in range(3):
+                bg.uniform[k][c] = float(vals[min(c, len(vals) - 1)])
+    return bg
 
 
 class ShaderGroup:
@@ -138,18 +172,7 @@ class ShadingSystem:
              stream=None):
         """Prepares the argument block once; returns a callable that
         launches the batched execution (repeated launches reuse it)."""
-        bg = BatchedGlobals()
-        bg.n = n
-        for k, name in enumerate(GLOBALS):
-            v = globals_.get(name)
-            if v is None:
-                continue
-            if hasattr(v, "data_ptr"):
-                bg.planes[k] = v.data_ptr()
-            else:
-                vals = list(v) if hasattr(v, "__len__") else [v]
-                for c in range(3):
-                    bg.uniform[k][c] = float(vals[min(c, len(vals) - 1)])
+        bg = batched_globals(n, globals_)
         import torch
         sp = ctypes.c_void_p(stream.cuda_stream if stream is not None
                              else torch.cuda.current_stream().cuda_stream)
@@ -191,3 +214,88 @@ def build_layered3(ss=None, outputs=()):
     ss.ShaderGroupEnd(g)
     ss.set_outputs(g, list(outputs))
     return ss, g
+
+
+def build_generated_group(ss, spec, name, outputs=("fout", "cout")):
+    """A group from groupgen's (layers, connections, ...) description:
+    layers [(shadername, layername, oso_text)], the last one the surface."""
+    layers, conns = spec[0], spec[1]
+    for sn, _, txt in layers:
+        ss.LoadMemoryCompiledShader(sn, txt)
+    g = ss.ShaderGroupBegin(name)
+    for k, (sn, ln, _) in enumerate(layers):
+        ss.Shader(g, "surface" if k == len(layers) - 1 else "shader", sn, ln)
+    for c in conns:
+        ss.ConnectShaders(g, *c)
+    ss.ShaderGroupEnd(g)
+    ss.set_outputs(g, list(outputs))
+    return g
+
+
+class GroupBatch:
+    """Many groups shaded by one persistent launch (oslmi_batch_*).
+
+    bind() takes one item per group: (n, globals dict, outputs, wo, wi,
+    eval_out) with the meaning ShadingSystem.execute gives them."""
+
+    def __init__(self, ss, groups, groups_per_kernel=0, load=True):
+        L = lib()
+        self.ss, self.groups = ss, list(groups)
+        arr = (ctypes.c_void_p * len(self.groups))(*[g.h for g in self.groups])
+        self.h = L.oslmi_batch_create(ss.h, len(self.groups), arr, groups_per_kernel,
+                                      1 if load else 0)
+        if not self.h:
+            raise RuntimeError("oslmi_batch_create: %s" % L.oslmi_last_error().decode())
+        self.jit_seconds = L.oslmi_batch_jit_seconds(self.h)
+        self.kernels = L.oslmi_batch_kernels(self.h)
+        self._keep = None
+
+    def __del__(self):
+        try:
+            lib().oslmi_batch_destroy(self.h)
+        except Exception:
+            pass
+
+    def bind(self, items):
+        arr = (BatchItem * len(items))()
+        for k, (n, glob, outputs, wo, wi, ev) in enumerate(items):
+            arr[k].globals = batched_globals(n, glob)
+            arr[k].d_outputs = outputs.data_ptr() if outputs is not None else None
+            arr[k].d_wo = wo.data_ptr() if wo is not None else None
+            arr[k].d_wi = wi.data_ptr() if wi is not None else None
+            arr[k].d_eval = ev.data_ptr() if ev is not None else None
+        check(lib().oslmi_batch_bind(self.h, len(items), arr))
+        self._keep = (arr, items)
+
+    @staticmethod
+    def _stream(stream):
+        import torch
+        s = stream or torch.cuda.current_stream()
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def set_config(self, **kw):
+        c = self.config()
+        for k, v in kw.items():
+            setattr(c, k, int(v))
+        check(lib().oslmi_batch_set_config(self.h, ctypes.byref(c)))
+
+    def config(self):
+        c = BatchConfig()
+        check(lib().oslmi_batch_get_config(self.h, ctypes.byref(c)))
+        return c
+
+    def launch(self, stream=None):
+        check(lib().oslmi_batch_launch(self.h, self._stream(stream)))
+
+    def launch_per_group(self, stream=None):
+        check(lib().oslmi_batch_launch_per_group(self.h, self._stream(stream)))
+
+    def capture(self, stream):
+        check(lib().oslmi_batch_capture(self.h, self._stream(stream)))
+
+    def autotune(self, stream=None, reps=3):
+        c = BatchConfig()
+        ms = ctypes.c_double()
+        check(lib().oslmi_batch_autotune(self.h, self._stream(stream), reps, ctypes.byref(c),
+                                         ctypes.byref(ms)))
+        return c.as_dict(), ms.value

@@ -10,6 +10,7 @@
 #include <thread>
 #include <vector>
 
+#include "closure.h"
 #include "noise.h"
 #include "omath.h"
 
@@ -197,6 +198,36 @@ void oref_math(int fn, const float* in, float* out, int64_t n)
         case 2: ofast_sincosf(in[i], &s, &c); out[i] = c; break;
         default: out[i] = ofast_cbrtf(in[i]); break;
         }
+    }
+}
+
+// CompositeBSDF prepare + eval over one lobe list for n direction pairs.
+// lobes: nlobes rows of 9 floats (kind, weight rgb, N xyz, alpha, eta);
+// wo, wi: 3 planes of n; out: 4 planes (weight rgb, pdf).
+void oref_composite_eval(int nlobes, const float* lobes, const float* wo, const float* wi,
+                         float* out, int64_t n)
+{
+    Composite c;
+    for (int k = 0; k < nlobes; ++k) {
+        const float* r = lobes + 9 * k;
+        Lobe l;
+        l.kind   = (int)r[0];
+        l.weight = V3(r[1], r[2], r[3]);
+        l.N      = V3(r[4], r[5], r[6]);
+        l.alpha  = r[7];
+        l.eta    = r[8];
+        c.add(l);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        Composite ci = c;
+        const V3 o(wo[i], wo[n + i], wo[2 * n + i]);
+        const V3 w(wi[i], wi[n + i], wi[2 * n + i]);
+        ci.prepare(o);
+        const BsdfSample s = ci.eval(o, w);
+        out[i]             = s.weight.x;
+        out[n + i]         = s.weight.y;
+        out[2 * n + i]     = s.weight.z;
+        out[3 * n + i]     = s.pdf;
     }
 }
 

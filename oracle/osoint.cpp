@@ -242,6 +242,7 @@ struct Group {
     std::vector<Layer> layers;
     std::vector<Conn> conns;
     ColorSys cs;
+    std::string fout = "fout", cout = "cout";  // last-layer outputs read by run
 };
 
 struct Exec {
@@ -704,6 +705,15 @@ int oref_group_connect(void* h, const char* srclayer, const char* srcparam, cons
     return 0;
 }
 
+// Names the last layer's float and triple outputs that run() reports
+// (default "fout" and "cout"; a name the layer lacks reads as zero).
+void oref_group_outputs(void* h, const char* fname, const char* cname)
+{
+    Group* g = (Group*)h;
+    g->fout  = fname ? fname : "";
+    g->cout  = cname ? cname : "";
+}
+
 // Runs the group over n points. Globals P, N (3 planes each); outputs of
 // the last layer: fout (1 plane) and cout (3 planes) into out4; with wo/wi
 // non-null, the closure Ci evaluated at (wo, wi) into eval4 (R, G, B, pdf).
@@ -712,16 +722,16 @@ void oref_group_run(void* h, const float* P, const float* N, const float* wo, co
 {
     Group* g = (Group*)h;
     const int last = (int)g->layers.size() - 1;
-    const int sf   = g->layers[last].m->find("fout");
-    const int sc   = g->layers[last].m->find("cout");
+    const int sf   = g->layers[last].m->find(g->fout);
+    const int sc   = g->layers[last].m->find(g->cout);
     auto work = [&](int64_t b, int64_t e) {
         Exec ex(*g);
         for (int64_t i = b; i < e; ++i) {
             ex.P = V3(P[i], P[n + i], P[2 * n + i]);
             ex.N = V3(N[i], N[n + i], N[2 * n + i]);
             ex.run();
-            out4[i]         = ex.F(last, sf);
-            const V3 c      = ex.T(last, sc);
+            out4[i]         = sf >= 0 ? ex.F(last, sf) : 0.0f;
+            const V3 c      = sc >= 0 ? ex.T(last, sc) : V3(0.0f);
             out4[n + i]     = c.x;
             out4[2 * n + i] = c.y;
             out4[3 * n + i] = c.z;

@@ -240,6 +240,12 @@ class Group:
         except Exception:
             pass
 
+    def outputs(self, fname, cname):
+        """Which last-layer float / triple outputs run() reports."""
+        self.L.oref_group_outputs.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]
+        self.L.oref_group_outputs(self.h, fname.encode(), cname.encode())
+        return self
+
     def run(self, P, N, wo=None, wi=None, nthreads=1):
         a = [np.ascontiguousarray(x, np.float32) for x in (P, N)]
         n = a[0].shape[1]
@@ -250,3 +256,18 @@ class Group:
                               _p(w[1]) if w else None, n, _p(out), _p(ev) if w else None,
                               nthreads)
         return out, ev
+
+
+def composite_eval(lobes, wo, wi):
+    """CompositeBSDF prepare + eval of one lobe list at n direction pairs.
+    lobes: rows (kind, w_r, w_g, w_b, Nx, Ny, Nz, alpha, eta); kind 0
+    diffuse, 1 GGX reflection. Returns 4 planes (weight rgb, pdf)."""
+    L = lib()
+    L.oref_composite_eval.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    lo = np.ascontiguousarray(lobes, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (wo, wi)]
+    n = a[0].shape[1]
+    out = np.zeros((4, n), np.float32)
+    L.oref_composite_eval(lo.shape[0], _p(lo), _p(a[0]), _p(a[1]), _p(out), n)
+    return out

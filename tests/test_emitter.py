@@ -81,3 +81,34 @@ def test_unsupported_op_reports_error():
     ss.ShaderGroupEnd(g)
     with pytest.raises(OSLMIError, match="unsupported op"):
         ss.emit(g)
+
+
+def test_batch_kernel_emits_and_compiles(tmp_path):
+    """A batch over two benchmark groups and two generated ones compiles
+    for gfx950 without a device; bad configs are rejected on the host."""
+    from openshadinglanguage_amd import groupgen as G
+    ss = S.ShadingSystem(jit_cache=str(tmp_path))
+    _, g1 = S.build_fbm_cell(ss)
+    _, g2 = S.build_layered3(ss)
+    specs, _ = G.make_groups(4, 3)
+    g3 = S.build_generated_group(ss, specs[1], "gen1")
+    g4 = S.build_generated_group(ss, specs[3], "gen3")
+    b = S.GroupBatch(ss, [g1, g2, g3, g4], load=False)
+    assert b.jit_seconds >= 0
+    assert len([f for f in os.listdir(tmp_path) if f.endswith(".hsaco")]) == 1
+    assert b.config().as_dict() == dict(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0)
+    for bad in (dict(block=96), dict(ppt=0), dict(ppt=9), dict(blocks_per_cu=0), dict(order=2),
+                dict(bb_lds=3)):
+        with pytest.raises(OSLMIError):
+            b.set_config(**bad)
+    b.set_config(block=64, ppt=4)
+    assert b.config().block == 64 and b.config().ppt == 4
+
+
+def test_batch_rejects_open_group():
+    ss = S.ShadingSystem(jit_cache=None)
+    ss.load_shader_file("fbm_cell")
+    g = ss.ShaderGroupBegin("open")
+    ss.Shader(g, "surface", "fbm_cell", "a")
+    with pytest.raises(RuntimeError):
+        S.GroupBatch(ss, [g], load=False)
