@@ -151,6 +151,88 @@ def bench_closure(steps, warmup, cpu, cpu_passes):
     return res
 
 
+def bench_batch(steps, warmup, cpu, autotune_reps=2):
+    """The 64 generated groups (seed 3, log-uniform point counts): one
+    persistent batch launch (autotuned, replayed from a hipGraph) against
+    the same kernels launched group after group, as the reference's
+    one-execute-per-group loop runs them."""
+    import torch
+    from openshadinglanguage_amd import groupgen as G
+    from openshadinglanguage_amd import shadingsys as S
+    dev = torch.device("cuda:0")
+    specs, counts = G.make_groups(64, 3)
+    ss = S.ShadingSystem()
+    groups = [S.build_generated_group(ss, specs[k], "gen%d" % k) for k in range(64)]
+    b = S.GroupBatch(ss, groups)
+    items, host = [], []
+    for k, n in enumerate(counts):
+        n = int(n)
+        w = G.group_inputs(n, 1000 + k)
+        t = {x: torch.from_numpy(w[x]).to(dev) for x in ("P", "N", "wo", "wi")}
+        glob = {"P": t["P"], "N": t["N"], "dPdx": (0.0, 0.0, 0.0), "dPdy": (0.0, 0.0, 0.0)}
+        items.append((n, glob, torch.empty((4, n), device=dev), t["wo"], t["wi"],
+                      torch.empty((4, n), device=dev)))
+        host.append(w)
+    total = int(counts.sum())
+    b.bind(items)
+    stream = torch.cuda.current_stream()
+    seq_ms, _ = _timed(lambda: b.launch_per_group(stream), stream, max(3, steps // 5), 1)
+    b.set_config(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0, streams=1)
+    untuned_ms, _ = _timed(lambda: b.launch(stream), stream, max(3, steps // 5), 1)
+    t0 = time.perf_counter()
+    best, tuner_ms = b.autotune(stream, reps=autotune_reps)
+    tune_s = time.perf_counter() - t0
+    side = torch.cuda.Stream()
+    b.capture(side)
+    kernel_ms, ms_per_step = _timed(lambda: b.launch(side), side, steps, warmup)
+    info = json.load(open(os.path.join(ROOT, "openshadinglanguage_amd",
+                                       "roofline_ops.json")))["batch64"]
+    achieved = info["ops_total"] / (kernel_ms / 1e3)
+    res = {"workload": "64 generated groups (seed 3): 2-12 layers of perlin/simplex/cell/gabor/"
+                       "hash noise, matrix transforms, color ops, data-dependent if/loops, "
+                       "closure sums; point counts log-uniform in [32, 262144]; outputs "
+                       "fout, cout + 1 closure eval per point",
+           "groups": 64, "points": total, "kernels": b.kernels,
+           "Mpts_per_s": round(total / (ms_per_step / 1e3) / 1e6, 2),
+           "ms_per_step": round(ms_per_step, 4), "graph_ms": round(kernel_ms, 4),
+           "untuned_ms": round(untuned_ms, 4), "per_group_launches_ms": round(seq_ms, 4),
+           "speedup_vs_per_group": round(seq_ms / kernel_ms, 2),
+           "autotune": {"best": best, "best_ms": round(tuner_ms, 4), "seconds": round(tune_s, 2)},
+           "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                        "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
+                        "frac": round(achieved / VALU_PEAK_OPS, 4), "traffic": None,
+                        "ops_per_point": round(info["ops_per_point"], 1),
+                        "kernel_ms": round(kernel_ms, 4)}}
+    if cpu:
+        # the oracle's OSO interpreter, group after group, on a bounded
+        # sample of every group; the full-workload time is extrapolated
+        # per group from its sample
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        nthreads = min(16, os.cpu_count() or 1)
+        est, sampled, wall = 0.0, 0, 0.0
+        for k, spec in enumerate(specs):
+            m = min(int(counts[k]), 1024)
+            w = host[k]
+            g = O.Group(spec[0], spec[1])
+            P, N = w["P"][:, :m].copy(), w["N"][:, :m].copy()
+            wo, wi = w["wo"][:, :m].copy(), w["wi"][:, :m].copy()
+            t0 = time.perf_counter()
+            g.run(P, N, wo, wi, nthreads=nthreads)
+            dt = time.perf_counter() - t0
+            wall += dt
+            sampled += m
+            est += dt * int(counts[k]) / m
+        res["cpu_baseline"] = {"value": round(total / est / 1e6, 4), "unit": "Mpts/s",
+                               "cores": nthreads, "kind": "port",
+                               "sample": "oracle OSO interpreter (not a compiled path), groups "
+                                         "one by one, first min(n, 1024) points of each "
+                                         "(%d points, %.1f s), extrapolated per group"
+                                         % (sampled, wall)}
+        res["gpu_over_cpu"] = round(res["Mpts_per_s"] / res["cpu_baseline"]["value"], 1)
+    return res
+
+
 SURROGATE_TRAIN = 1 << 24   # training samples (seed 5)
 SURROGATE_CALIB = 1 << 20   # calibration queries (seed 7)
 SURROGATE_QUERIES = 1 << 23  # benchmark queries (seed 6)
@@ -265,6 +347,8 @@ def main():
                     help="skip the layered-closure measurement")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate training + measurement")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-group batch measurement")
     ap.add_argument("--train-steps", type=int, default=12000)
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -380,6 +464,8 @@ def main():
         cpu_inp = inp if name == "fbm_cell" else inp
         line["cpu_baseline"] = cpu_baseline(name, cpu_inp, nthreads, args.cpu_passes)
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    if not args.no_batch:
+        line["batch"] = bench_batch(max(5, args.steps // 2), args.warmup, not args.no_cpu)
     if not args.no_closure:
         line["closure"] = bench_closure(max(5, args.steps // 2), args.warmup, not args.no_cpu,
                                         args.cpu_passes)

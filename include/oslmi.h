@@ -138,10 +138,12 @@ typedef struct oslmi_batch_item {
  * per tile (1..8; a tile is block * ppt points), workgroups per CU of the
  * persistent grid (1..32), blackbody table in LDS (1) or read through the
  * caches (0), tile order (0: groups in item order, 1: groups by measured
- * cost per point, most expensive first; both keep each group's tiles
- * together). */
+ * cost per tile, most expensive first; both keep each group's tiles
+ * together), streams the batch's kernels are spread over (1..4; more than
+ * one lets the kernels overlap, forked from and joined back into the
+ * caller's stream). */
 typedef struct oslmi_batch_config {
-    int block, ppt, blocks_per_cu, bb_lds, order;
+    int block, ppt, blocks_per_cu, bb_lds, order, streams;
 } oslmi_batch_config;
 /* Emits and compiles (hiprtc, code-object cache) the batch's persistent
  * kernels over ngroups ended groups: groups_per_kernel groups per kernel

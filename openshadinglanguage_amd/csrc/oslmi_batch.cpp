@@ -18,10 +18,11 @@
 // workgroups per CU pulls tiles from a device counter.
 //
 // The launch shape is tunable (block, ppt, blocks_per_cu, blackbody table
-// in LDS or not, tile order); oslmi_batch_autotune times every candidate on
-// the bound work and keeps the fastest. oslmi_batch_capture records the
-// launches into a hipGraph; every counter resets itself at the end of its
-// launch, so replays need no memset node.
+// in LDS or not, tile order, number of streams the kernels overlap on);
+// oslmi_batch_autotune times candidates on the bound work and keeps the
+// fastest. oslmi_batch_capture records the launches into a hipGraph; every
+// counter resets itself at the end of its launch, so replays need no
+// memset node.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,7 +76,12 @@ struct Batch {
     size_t counter_cap = 0;
     const float* bb    = nullptr;
     std::vector<double> cost;  // measured ms per tile per item (order 1)
-    oslmi_batch_config cfg{ 256, 1, 8, 1, 0 };
+    oslmi_batch_config cfg{ 256, 1, 8, 1, 0, 1 };
+    // side streams the kernels are spread over (cfg.streams > 1), with the
+    // fork event and one join event per side stream
+    std::vector<hipStream_t> side;
+    hipEvent_t fork_ev = nullptr;
+    std::vector<hipEvent_t> join_ev;
     // captured launches
     hipGraph_t graph         = nullptr;
     hipGraphExec_t gexec     = nullptr;
@@ -84,6 +90,12 @@ struct Batch {
     ~Batch()
     {
         drop_graph();
+        for (hipStream_t s : side)
+            (void)hipStreamDestroy(s);
+        for (hipEvent_t e : join_ev)
+            (void)hipEventDestroy(e);
+        if (fork_ev)
+            (void)hipEventDestroy(fork_ev);
         if (dargs)
             (void)hipFree(dargs);
         if (dtiles)
@@ -211,12 +223,56 @@ struct Batch {
                          "hipModuleLaunchKernel(oslmi_batch_main)");
     }
 
+    int ensure_side_streams(int ns)
+    {
+        if (!fork_ev)
+            if (int rc = check_hip(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate"))
+                return rc;
+        while ((int)side.size() < ns) {
+            hipStream_t s = nullptr;
+            hipEvent_t e  = nullptr;
+            // default priority, non-blocking with respect to the null stream
+            if (int rc = check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"))
+                return rc;
+            side.push_back(s);
+            if (int rc = check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"))
+                return rc;
+            join_ev.push_back(e);
+        }
+        return OSLMI_OK;
+    }
+
+    // Every part's kernel over its tiles. With cfg.streams > 1 the kernels
+    // are dealt round-robin to side streams that wait for the caller's
+    // stream and are waited for by it (a fork/join that stream capture
+    // records as parallel graph branches).
     int launch_all(hipStream_t stream)
     {
         const int ni = (int)hargs.size();
-        for (size_t q = 0; q < parts.size(); ++q)
-            if (int rc = launch_range(parts[q], parts[q].first_tile, parts[q].ntiles, ni + (int)q, stream))
+        const int ns = std::min<int>(cfg.streams, (int)parts.size());
+        if (ns <= 1) {
+            for (size_t q = 0; q < parts.size(); ++q)
+                if (int rc = launch_range(parts[q], parts[q].first_tile, parts[q].ntiles, ni + (int)q, stream))
+                    return rc;
+            return OSLMI_OK;
+        }
+        if (int rc = ensure_side_streams(ns))
+            return rc;
+        if (int rc = check_hip(hipEventRecord(fork_ev, stream), "hipEventRecord(fork)"))
+            return rc;
+        for (int s = 0; s < ns; ++s)
+            if (int rc = check_hip(hipStreamWaitEvent(side[s], fork_ev, 0), "hipStreamWaitEvent(fork)"))
                 return rc;
+        for (size_t q = 0; q < parts.size(); ++q)
+            if (int rc = launch_range(parts[q], parts[q].first_tile, parts[q].ntiles, ni + (int)q,
+                                      side[q % ns]))
+                return rc;
+        for (int s = 0; s < ns; ++s) {
+            if (int rc = check_hip(hipEventRecord(join_ev[s], side[s]), "hipEventRecord(join)"))
+                return rc;
+            if (int rc = check_hip(hipStreamWaitEvent(stream, join_ev[s], 0), "hipStreamWaitEvent(join)"))
+                return rc;
+        }
         return OSLMI_OK;
     }
 
@@ -246,6 +302,8 @@ int validate(const oslmi_batch_config& c)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch: bb_lds must be 0 or 1");
     if (c.order != 0 && c.order != 1)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch: order must be 0 or 1");
+    if (c.streams < 1 || c.streams > 4)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: streams must be in 1..4");
     return OSLMI_OK;
 }
 
@@ -529,7 +587,7 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_con
     // cost of every item alone (block 256, one point per work-item), as ms
     // per tile with as many of its tiles resident as fit
     {
-        b.cfg = oslmi_batch_config{ 256, 1, 32, 1, 0 };
+        b.cfg = oslmi_batch_config{ 256, 1, 32, 1, 0, 1 };
         if (int rc = b.build_tiles())
             return rc;
         const int ni = (int)b.hargs.size();
@@ -546,8 +604,25 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_con
         }
         b.cost = cost;
     }
-    oslmi_batch_config win{ 256, 1, 8, 1, 0 };
+    // two stages: the launch geometry (block, ppt, blocks_per_cu) with the
+    // LDS table, cost order and all side streams; then those three
+    // switches with the winning geometry
+    const int max_streams = std::min<int>(4, (int)b.parts.size());
+    oslmi_batch_config win{ 256, 1, 8, 1, 1, max_streams };
     double win_ms = 1e30;
+    auto trial = [&](const oslmi_batch_config& c) -> int {
+        b.cfg = c;
+        if (int rc = b.build_tiles())
+            return rc;
+        double ms = 0.0;
+        if (int rc = time_ms(st, reps, [&] { return b.launch_all(st); }, ms))
+            return rc;
+        if (ms < win_ms) {
+            win_ms = ms;
+            win    = c;
+        }
+        return OSLMI_OK;
+    };
     for (int block : { 64, 128, 256 }) {
         int maxb = 1;
         for (const Part& p : b.parts)
@@ -559,20 +634,16 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_con
         bpcs.push_back(maxb);
         for (int ppt : { 1, 2, 4 })
             for (int bpc : bpcs)
-                for (int lds = 0; lds <= (b.any_bb ? 1 : 0); ++lds)
-                    for (int order = 0; order <= 1; ++order) {
-                        b.cfg = oslmi_batch_config{ block, ppt, bpc, b.any_bb ? lds : 1, order };
-                        if (int rc = b.build_tiles())
-                            return rc;
-                        double ms = 0.0;
-                        if (int rc = time_ms(st, reps, [&] { return b.launch_all(st); }, ms))
-                            return rc;
-                        if (ms < win_ms) {
-                            win_ms = ms;
-                            win    = b.cfg;
-                        }
-                    }
+                if (int rc = trial(oslmi_batch_config{ block, ppt, bpc, 1, 1, max_streams }))
+                    return rc;
     }
+    const oslmi_batch_config geo = win;
+    for (int lds = 0; lds <= (b.any_bb ? 1 : 0); ++lds)
+        for (int order = 0; order <= 1; ++order)
+            for (int ns = 1; ns <= max_streams; ns *= 2)
+                if (int rc = trial(oslmi_batch_config{ geo.block, geo.ppt, geo.blocks_per_cu,
+                                                       b.any_bb ? lds : 1, order, ns }))
+                    return rc;
     b.cfg = win;
     if (int rc = b.build_tiles())
         return rc;

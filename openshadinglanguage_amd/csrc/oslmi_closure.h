@@ -30,15 +30,31 @@ struct Lobe {
     float eta;
 };
 
+// Every access below uses a compile-time slot index (loops are unrolled
+// and guarded by the run-time count), so a list lives in registers; a
+// run-time index would put it in scratch memory.
 template<int MAXL> struct LobeList {
     int n;
     Lobe l[MAXL];
     float pdf[MAXL];
     OSLMI_DEV LobeList() : n(0) {}
+    // copy of a list of another capacity (lobes beyond MAXL are dropped,
+    // as add() drops them)
+    template<int K2> OSLMI_DEV LobeList(const LobeList<K2>& o) : n(0)
+    {
+#pragma unroll
+        for (int k = 0; k < (K2 < MAXL ? K2 : MAXL); ++k)
+            if (k < o.n)
+                l[k] = o.l[k];
+        n = o.n < MAXL ? o.n : MAXL;
+    }
     OSLMI_DEV void add(const Lobe& x)
     {
-        if (n < MAXL)
-            l[n++] = x;
+#pragma unroll
+        for (int k = 0; k < MAXL; ++k)
+            if (k == n)
+                l[k] = x;
+        n = n < MAXL ? n + 1 : n;
     }
 };
 
@@ -146,13 +162,17 @@ template<int MAXL> OSLMI_DEV void composite_prepare(LobeList<MAXL>& L, V3 wo)
 {
     const V3 pw(1.0f);
     float total = 0;
-    for (int i = 0; i < L.n; i++) {
-        L.pdf[i] = dot(L.l[i].weight, pw * lobe_albedo(L.l[i], wo)) / (pw.x + pw.y + pw.z);
-        total += L.pdf[i];
-    }
+#pragma unroll
+    for (int i = 0; i < MAXL; i++)
+        if (i < L.n) {
+            L.pdf[i] = dot(L.l[i].weight, pw * lobe_albedo(L.l[i], wo)) / (pw.x + pw.y + pw.z);
+            total += L.pdf[i];
+        }
     if (total > 0 || total > 1)
-        for (int i = 0; i < L.n; i++)
-            L.pdf[i] /= total;
+#pragma unroll
+        for (int i = 0; i < MAXL; i++)
+            if (i < L.n)
+                L.pdf[i] /= total;
 }
 
 OSLMI_DEV void mis_update(V3* w, float* pdf, V3 ow, float opdf, float b)
@@ -177,10 +197,12 @@ template<int MAXL> OSLMI_DEV BsdfEval composite_eval(const LobeList<MAXL>& L, V3
     BsdfEval s;
     s.weight = V3(0.0f);
     s.pdf    = 0.0f;
-    for (int i = 0; i < L.n; i++) {
-        BsdfEval b = lobe_eval(L.l[i], wo, wi);
-        mis_update(&s.weight, &s.pdf, b.weight * L.l[i].weight, b.pdf, L.pdf[i]);
-    }
+#pragma unroll
+    for (int i = 0; i < MAXL; i++)
+        if (i < L.n) {
+            BsdfEval b = lobe_eval(L.l[i], wo, wi);
+            mis_update(&s.weight, &s.pdf, b.weight * L.l[i].weight, b.pdf, L.pdf[i]);
+        }
     return s;
 }
 

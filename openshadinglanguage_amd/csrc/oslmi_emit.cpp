@@ -219,6 +219,36 @@ struct Emitter {
         return d.count(o) > 0;
     }
 
+    // Lobe capacity of the group's closure values: the number of closure
+    // constructors (each makes one lobe and ADD only concatenates), or the
+    // full 8 when a closure is written inside a loop.
+    int closure_capacity() const
+    {
+        int count = 0;
+        bool in_loop_write = false;
+        for (size_t L = 0; L < g.layers.size(); ++L) {
+            const std::vector<OOp>& ops = M((int)L).ops;
+            std::vector<char> inloop(ops.size(), 0);
+            for (size_t i = 0; i < ops.size(); ++i)
+                if ((ops[i].name == "for" || ops[i].name == "while" || ops[i].name == "dowhile")
+                    && ops[i].jump[3] > (int)i)
+                    for (int j = (int)i + 1; j < ops[i].jump[3] && j < (int)ops.size(); ++j)
+                        inloop[j] = 1;
+            for (size_t i = 0; i < ops.size(); ++i) {
+                const OOp& op = ops[i];
+                if (op.name == "closure")
+                    ++count;
+                if (inloop[i] && !op.args.empty() && op.args[0] >= 0
+                    && op.args[0] < (int)M((int)L).syms.size()
+                    && S((int)L, op.args[0]).type.base == kClosure)
+                    in_loop_write = true;
+            }
+        }
+        if (in_loop_write || count > 8)
+            return 8;
+        return count < 1 ? 1 : count;
+    }
+
     void analyze()
     {
         deriv.assign(g.layers.size(), {});
@@ -1093,6 +1123,7 @@ struct Emitter {
         k << "// Emitted by the oslmi group emitter for group '" << g.name << "'.\n";
         k << prelude();
         k << "extern \"C\" __global__ __launch_bounds__(256) void oslmi_group_main(GroupArgs A)\n{\n";
+        k << "    using Closure = LobeList<" << closure_capacity() << ">;\n";
         if (uses_cs)
             k << "    const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
         if (uses_bb) {
@@ -1167,6 +1198,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
         std::ostringstream c;
         c << "    case " << gi << ": {  // group '" << groups[j]->name << "'\n";
         c << "        const GroupArgs& A = args[" << gi << "];\n";
+        c << "        using Closure = LobeList<" << e.closure_capacity() << ">;\n";
         if (e.uses_cs)
             c << "        const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
         c << "        for (;;) {\n";

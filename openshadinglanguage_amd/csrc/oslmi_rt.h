@@ -207,39 +207,46 @@ OSLMI_DEV V3 xform_vector(const M44& M, V3 p)
 }
 
 // ---------------------------------------------------------- closures
+// A closure value is a LobeList whose capacity the emitter bounds per
+// group (the number of closure constructors it can reach, at most 8);
+// primitives make one-lobe lists, ADD concatenates, MUL scales.
 typedef LobeList<8> Closure;
-OSLMI_DEV Closure closure_none() { return Closure(); }
-OSLMI_DEV Closure closure_diffuse(V3 N)
+OSLMI_DEV LobeList<1> closure_diffuse(V3 N)
 {
-    Closure c;
+    LobeList<1> c;
     Lobe l;
     l.id = kClosureDiffuse; l.weight = V3(1.0f); l.N = N; l.alpha = 0.0f; l.eta = 0.0f;
     c.add(l);
     return c;
 }
-OSLMI_DEV Closure closure_microfacet_ggx(V3 N, float alpha, float eta)
+OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
 {
-    Closure c;
+    LobeList<1> c;
     Lobe l;
     l.id = kClosureMicrofacetGGX; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
     c.add(l);
     return c;
 }
 // color * closure (a MUL node): scales every lobe weight
-OSLMI_DEV Closure mul(const Closure& c, V3 w)
+template<int K> OSLMI_DEV LobeList<K> mul(const LobeList<K>& c, V3 w)
 {
-    Closure r = c;
-    for (int i = 0; i < r.n; ++i)
-        r.l[i].weight = r.l[i].weight * w;
+    LobeList<K> r = c;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (i < r.n)
+            r.l[i].weight = r.l[i].weight * w;
     return r;
 }
-OSLMI_DEV Closure mul(V3 w, const Closure& c) { return mul(c, w); }
+template<int K> OSLMI_DEV LobeList<K> mul(V3 w, const LobeList<K>& c) { return mul(c, w); }
 // ADD node: lobes of a then b (depth-first order of process_bsdf_closure)
-OSLMI_DEV Closure add(const Closure& a, const Closure& b)
+template<int KA, int KB>
+OSLMI_DEV LobeList<(KA + KB < 8 ? KA + KB : 8)> add(const LobeList<KA>& a, const LobeList<KB>& b)
 {
-    Closure r = a;
-    for (int i = 0; i < b.n; ++i)
-        r.add(b.l[i]);
+    LobeList<(KA + KB < 8 ? KA + KB : 8)> r(a);
+#pragma unroll
+    for (int i = 0; i < KB; ++i)
+        if (i < b.n)
+            r.add(b.l[i]);
     return r;
 }
 

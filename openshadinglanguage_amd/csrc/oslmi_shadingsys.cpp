@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -131,7 +132,10 @@ int build_code_object(const ShadingSys& ss, const std::string& src, const std::s
         hiprtcDestroyProgram(&prog);
         if (!cache_path.empty()) {
             mkdirs(ss.cache_dir);
-            std::string tmp = cache_path + ".tmp" + std::to_string((long)getpid());
+            // unique per process and per call (compiles may run in threads)
+            static std::atomic<long> seq { 0 };
+            std::string tmp = cache_path + ".tmp" + std::to_string((long)getpid()) + "_"
+                              + std::to_string(seq.fetch_add(1));
             std::ofstream f(tmp, std::ios::binary);
             f.write(code.data(), (std::streamsize)code.size());
             f.close();

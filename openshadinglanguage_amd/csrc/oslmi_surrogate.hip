@@ -640,6 +640,114 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
     }
 }
 
+// The network over net[0 .. *netn) with the weights in LDS: every
+// workgroup (8 waves) copies the packed fragments and bias initialisers
+// (48 KB) to LDS once; MFMA A operands are then read from LDS as each
+// layer needs them. With the weights out of the register file a wave
+// needs about a third of the registers, so four waves share a SIMD and one
+// wave's SiLU work overlaps another's MFMAs. TILES independent 16-query
+// tiles per iteration share every weight read.
+constexpr int INFER_BLOCK = 512;
+template<int TILES>
+__global__ __launch_bounds__(INFER_BLOCK, 2) void k_infer_lds(InferConsts kc,
+                                                              const bf16x8* __restrict__ frag,
+                                                              const f32x4* __restrict__ biasp,
+                                                              const int* __restrict__ net,
+                                                              const int* __restrict__ netn,
+                                                              const float* __restrict__ q,
+                                                              int64_t n, float* __restrict__ out)
+{
+    __shared__ bf16x8 s_w[NFRAG * 64];
+    __shared__ f32x4 s_b[NBIAS * 64];
+    for (int j = threadIdx.x; j < NFRAG * 64; j += INFER_BLOCK)
+        s_w[j] = frag[j];
+    for (int j = threadIdx.x; j < NBIAS * 64; j += INFER_BLOCK)
+        s_b[j] = biasp[j];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int64_t cnt    = *netn;
+    const int64_t ngroup = (cnt + 16 * TILES - 1) / (16 * TILES);
+    const int64_t wave   = ((int64_t)blockIdx.x * INFER_BLOCK + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * INFER_BLOCK) >> 6;
+    float noff[8], nscl[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int k = 8 * g + j;
+        noff[j]     = k < NIN ? kc.nm.in_off[k < NIN ? k : 0] : 0.0f;
+        nscl[j]     = k < NIN ? kc.nm.in_scale[k < NIN ? k : 0] : 0.0f;
+    }
+    for (int64_t grp = wave; grp < ngroup; grp += nwaves) {
+        // keeps the (loop-invariant) LDS weight reads inside the loop: held
+        // across iterations they would need the registers this kernel frees
+        asm volatile("" ::: "memory");
+        int qi[TILES];
+        bf16x8 x[TILES];
+#pragma unroll
+        for (int u = 0; u < TILES; u++) {
+            const int64_t e = (grp * TILES + u) * 16 + r;
+            const bool ok   = e < cnt;
+            qi[u]           = ok ? net[e] : -1;
+            const int64_t qs = ok ? qi[u] : 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int k   = 8 * g + j;
+                const int kc_ = k < NIN ? k : NIN - 1;
+                const float v = q[kc_ * n + qs];
+                x[u][j]       = (bf16)(k < NIN && ok ? (v - noff[j]) * nscl[j] : 0.0f);
+            }
+        }
+        f32x4 a[TILES][4];
+        bf16x8 h0[TILES], h1[TILES];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const bf16x8 w  = s_w[t * 64 + lane];
+            const f32x4 c   = s_b[t * 64 + lane];
+#pragma unroll
+            for (int u = 0; u < TILES; u++)
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, x[u], c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < TILES; u++) {
+            h0[u] = act_pair(a[u][0], a[u][1]);
+            h1[u] = act_pair(a[u][2], a[u][3]);
+        }
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const bf16x8 w0 = s_w[(4 + l * 8 + t * 2) * 64 + lane];
+                const bf16x8 w1 = s_w[(4 + l * 8 + t * 2 + 1) * 64 + lane];
+                const f32x4 c   = s_b[((l + 1) * 4 + t) * 64 + lane];
+#pragma unroll
+                for (int u = 0; u < TILES; u++) {
+                    a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, h0[u], c, 0, 0, 0);
+                    a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, h1[u], a[u][t], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < TILES; u++) {
+                h0[u] = act_pair(a[u][0], a[u][1]);
+                h1[u] = act_pair(a[u][2], a[u][3]);
+            }
+        }
+        const bf16x8 w50 = s_w[28 * 64 + lane];
+        const bf16x8 w51 = s_w[29 * 64 + lane];
+        const f32x4 c5   = s_b[16 * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < TILES; u++) {
+            f32x4 y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w50, h0[u], c5, 0, 0, 0);
+            y       = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w51, h1[u], y, 0, 0, 0);
+            // lanes of group 0 hold outputs 0..3 of their query
+            if (g == 0 && qi[u] >= 0) {
+#pragma unroll
+                for (int o = 0; o < 4; o++)
+                    out[o * n + qi[u]] = expf(y[o] * kc.nm.out_sd[o] + kc.nm.out_mu[o]);
+            }
+        }
+    }
+}
+
 __global__ void k_zero_int(int* p) { *p = 0; }
 
 // ---- host context ----------------------------------------------------------
@@ -1030,8 +1138,20 @@ extern "C" int oslmi_surrogate_infer(oslmi_surrogate* s, const oslmi_layered_mat
     hipLaunchKernelGGL(k_route, dim3((unsigned)rblocks), dim3(256), 0, st, s->c.dom, s->c.cells,
                        mode, d_query, n, d_out, s->c.netlist, s->c.fbn, d_fallback_index,
                        d_fallback_count);
-    hipLaunchKernelGGL(k_infer, dim3(infer_grid(n)), dim3(256), 0, st, kc, s->c.frag, s->c.biasp,
-                       s->c.netlist, s->c.fbn, d_query, n, d_out);
+    const int variant = getenv("OSLMI_INFER_KERNEL") ? atoi(getenv("OSLMI_INFER_KERNEL")) : 0;
+    if (variant == 0) {
+        hipLaunchKernelGGL(k_infer, dim3(infer_grid(n)), dim3(256), 0, st, kc, s->c.frag,
+                           s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out);
+    } else {
+        int64_t blocks = (n + 16 * variant * 8 - 1) / (16 * variant * 8);
+        blocks         = std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 2));
+        if (variant == 4)
+            hipLaunchKernelGGL(k_infer_lds<4>, dim3((unsigned)blocks), dim3(INFER_BLOCK), 0, st, kc,
+                               s->c.frag, s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out);
+        else
+            hipLaunchKernelGGL(k_infer_lds<2>, dim3((unsigned)blocks), dim3(INFER_BLOCK), 0, st, kc,
+                               s->c.frag, s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out);
+    }
     if (int rc = check_launch("oslmi_surrogate_infer"))
         return rc;
     if (mode == 1)

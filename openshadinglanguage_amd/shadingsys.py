@@ -37,7 +37,7 @@ class BatchItem(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("ppt", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
-                ("bb_lds", ctypes.c_int), ("order", ctypes.c_int)]
+                ("bb_lds", ctypes.c_int), ("order", ctypes.c_int), ("streams", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

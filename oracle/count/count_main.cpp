@@ -20,6 +20,7 @@ thread_local uint64_t ocount::g_ops = 0;
 #include "../shade.cpp"
 #include "../mtx.cpp"
 #include "../mtx_api.cpp"
+#include "../osoint.cpp"
 #undef float
 #undef uint32_t
 

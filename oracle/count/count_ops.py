@@ -77,7 +77,36 @@ def main():
                            "points_counted": q.shape[1],
                            "sample": "32768 shading-frame queries (seed 6); 1 eval per query"}
     assert np.array_equal(out, O.mtx_eval_local(mat, q)), "counting build differs"
-    res["_doc"] = ("algorithmic 32-bit lane-ops per point from the op-counting oracle "
+    # the 64 generated groups (seed 3), fout/cout outputs + 1 closure eval
+    from openshadinglanguage_amd import groupgen as G
+    specs, counts = G.make_groups(64, 3)
+    L.oref_group_new.restype = P
+    L.oref_group_layer.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p]
+    L.oref_group_connect.argtypes = [P] + [ctypes.c_char_p] * 4
+    L.oref_group_run.argtypes = [P, P, P, P, P, ctypes.c_int64, P, P, ctypes.c_int]
+    L.oref_group_free.argtypes = [P]
+    per_group = []
+    for k, spec in enumerate(specs):
+        h = L.oref_group_new()
+        for _, ln, text in spec[0]:
+            assert L.oref_group_layer(h, text.encode(), ln.encode()) == 0
+        for c in spec[1]:
+            assert L.oref_group_connect(h, *[x.encode() for x in c]) == 0
+        w = G.group_inputs(512, 100 + k)
+        out = np.zeros((4, 512), np.float32)
+        ev = np.zeros((4, 512), np.float32)
+        L.ocount_reset()
+        L.oref_group_run(h, p(w["P"]), p(w["N"]), p(w["wo"]), p(w["wi"]), 512, p(out), p(ev), 1)
+        per_group.append(L.ocount_get() / 512)
+        L.oref_group_free(h)
+        ref_out, ref_ev = O.Group(spec[0], spec[1]).run(w["P"], w["N"], w["wo"], w["wi"])
+        assert np.array_equal(out, ref_out) and np.array_equal(ev, ref_ev), "counting build differs"
+    total = float(np.dot(per_group, counts.astype(np.float64)))
+    res["batch64"] = {"ops_per_point": total / float(counts.sum()), "ops_total": total,
+                      "points": int(counts.sum()), "ops_per_point_by_group": per_group,
+                      "sample": "512 points per group (group_inputs seed 100 + group); "
+                                "fout, cout and 1 closure eval per point"}
+    res["_doc"] =("algorithmic 32-bit lane-ops per point from the op-counting oracle "
                    "(oracle/count/count_ops.py)")
     dst = os.path.join(ROOT, "openshadinglanguage_amd", "roofline_ops.json")
     json.dump(res, open(dst, "w"), indent=1)

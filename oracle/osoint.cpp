@@ -384,8 +384,8 @@ struct Exec {
                 else if (key == "impulses") go.impulses = v;
                 else if (key == "direction") go.direction = T(L, op.args[a + 1]);
             }
-            D2<float> p[3] = { D2<float>(c[0]), D2<float>(dim > 1 ? c[1] : 0.0f),
-                               D2<float>(dim > 2 ? c[2] : 0.0f) };
+            D2<float> p[3] = { D2<float>(c[0]), D2<float>(dim > 1 ? c[1] : float(0.0f)),
+                               D2<float>(dim > 2 ? c[2] : float(0.0f)) };
             setF(L, r, gabor(dim, p, go).v);
         } else {
             fprintf(stderr, "osoint: noise '%s'\n", name.c_str());
@@ -730,7 +730,7 @@ void oref_group_run(void* h, const float* P, const float* N, const float* wo, co
             ex.P = V3(P[i], P[n + i], P[2 * n + i]);
             ex.N = V3(N[i], N[n + i], N[2 * n + i]);
             ex.run();
-            out4[i]         = sf >= 0 ? ex.F(last, sf) : 0.0f;
+            out4[i]         = sf >= 0 ? ex.F(last, sf) : float(0.0f);
             const V3 c      = sc >= 0 ? ex.T(last, sc) : V3(0.0f);
             out4[n + i]     = c.x;
             out4[2 * n + i] = c.y;

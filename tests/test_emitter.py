@@ -96,9 +96,10 @@ def test_batch_kernel_emits_and_compiles(tmp_path):
     b = S.GroupBatch(ss, [g1, g2, g3, g4], load=False)
     assert b.jit_seconds >= 0
     assert len([f for f in os.listdir(tmp_path) if f.endswith(".hsaco")]) == 1
-    assert b.config().as_dict() == dict(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0)
+    assert b.config().as_dict() == dict(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0,
+                                        streams=1)
     for bad in (dict(block=96), dict(ppt=0), dict(ppt=9), dict(blocks_per_cu=0), dict(order=2),
-                dict(bb_lds=3)):
+                dict(bb_lds=3), dict(streams=0), dict(streams=5)):
         with pytest.raises(OSLMIError):
             b.set_config(**bad)
     b.set_config(block=64, ppt=4)

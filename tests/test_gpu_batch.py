@@ -108,16 +108,19 @@ def test_batch_matches_standalone_kernels_full_counts(setup):
         np.testing.assert_array_equal(got[k][1], ev.cpu().numpy(), err_msg="group %d eval" % k)
 
 
-@pytest.mark.parametrize("cfg", [dict(block=64, ppt=1, blocks_per_cu=1, bb_lds=0, order=0),
-                                 dict(block=128, ppt=4, blocks_per_cu=2, bb_lds=1, order=0),
-                                 dict(block=256, ppt=2, blocks_per_cu=32, bb_lds=1, order=0)])
+@pytest.mark.parametrize("cfg", [dict(block=64, ppt=1, blocks_per_cu=1, bb_lds=0, order=0,
+                                      streams=1),
+                                 dict(block=128, ppt=4, blocks_per_cu=2, bb_lds=1, order=0,
+                                      streams=4),
+                                 dict(block=256, ppt=2, blocks_per_cu=32, bb_lds=1, order=1,
+                                      streams=2)])
 def test_launch_shapes_agree(setup, cfg):
     dev = torch.device("cuda:0")
     counts = [min(int(c), 5000) for c in setup["counts"]]
     items, _, outs = _items(counts, 2000, dev)
     b = setup["batch"]
     b.bind(items)
-    b.set_config(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0)
+    b.set_config(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0, streams=1)
     b.launch()
     torch.cuda.synchronize()
     ref = _cpu(outs)
@@ -170,7 +173,7 @@ def test_autotune_keeps_results(setup):
     items, _, outs = _items(counts, 4000, dev)
     b = setup["batch"]
     b.bind(items)
-    b.set_config(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0)
+    b.set_config(block=256, ppt=1, blocks_per_cu=8, bb_lds=1, order=0, streams=1)
     b.launch()
     torch.cuda.synchronize()
     ref = _cpu(outs)
