@@ -237,6 +237,8 @@ SURROGATE_TRAIN = 1 << 24   # training samples (seed 5)
 SURROGATE_CALIB = 1 << 20   # calibration queries (seed 7)
 SURROGATE_QUERIES = 1 << 23  # benchmark queries (seed 6)
 MLP_FLOP_PER_QUERY = 2 * (14 * 64 + 3 * 64 * 64 + 64 * 4)
+# predicted weights below this go to the exact closure (spot check)
+SPOT_CHECK_MIN_WEIGHT = 1e-3
 MFMA_BF16_PEAK = 2.5e15
 
 
@@ -278,8 +280,16 @@ def bench_surrogate(steps, warmup, train_steps, batch=16384, budget=0.01):
     pc = torch.empty_like(tc)
     s.infer(qc, pc)
     torch.cuda.synchronize()
-    table, _ = S.calibrate(pc.cpu().numpy(), tc.cpu().numpy(), qc.cpu().numpy(), budget)
+    ew_c, ep_c = S.relative_errors(pc.cpu().numpy(), tc.cpu().numpy())
+    table, _, predicted, pfb = S.calibrate(pc.cpu().numpy(), tc.cpu().numpy(), qc.cpu().numpy(),
+                                           budget, min_weight=SPOT_CHECK_MIN_WEIGHT)
+    calib = {"queries": SURROGATE_CALIB,
+             "network_mean_err": [float(ew_c.mean()), float(ep_c.mean())],
+             "predicted_mean_err_after_routing": list(predicted),
+             "predicted_fallback_fraction": pfb, "margin": 0.8,
+             "spot_check_min_weight": SPOT_CHECK_MIN_WEIGHT}
     s.set_routing(table)
+    s.set_spot_check(SPOT_CHECK_MIN_WEIGHT)
     del qc, tc, pc
 
     # benchmark queries from the closure workload's distribution
@@ -318,6 +328,7 @@ def bench_surrogate(steps, warmup, train_steps, batch=16384, budget=0.01):
         "network_only_ms": round(ms_net, 4),
         "fallback_fraction": round(nfb / n, 4),
         "routed_cells": int(table.sum()),
+        "calibration": calib,
         "rel_err_weight": {"mean": float(ew[answered].mean()),
                            "p99": float(np.quantile(ew[answered], 0.99))},
         "rel_err_pdf": {"mean": float(ep[answered].mean()),
@@ -349,7 +360,7 @@ def main():
                     help="skip the surrogate training + measurement")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-group batch measurement")
-    ap.add_argument("--train-steps", type=int, default=12000)
+    ap.add_argument("--train-steps", type=int, default=60000)
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")

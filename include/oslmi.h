@@ -275,6 +275,11 @@ int oslmi_surrogate_train_step(oslmi_surrogate* s, int64_t batch, const float* d
  * between L and the mirror direction of V); a non-zero cell sends its
  * queries to the exact closure. */
 int oslmi_surrogate_set_routing(oslmi_surrogate* s, const unsigned char* cells, int count);
+/* Spot check (with fallback only): a query whose predicted weight is below
+ * min_weight in any channel is answered by the exact closure instead (the
+ * network fits log values, so its relative error grows as values vanish).
+ * 0 disables it (the default). */
+int oslmi_surrogate_set_spot_check(oslmi_surrogate* s, float min_weight);
 /* Answers n queries (d_query 14 planes) into d_out (4 planes: weight rgb,
  * pdf). With m non-NULL, queries outside the training domain or in routed
  * cells are listed in d_fallback_index / *d_fallback_count (device, n

@@ -67,6 +67,7 @@ def _declare(L):
         "oslmi_surrogate_train_step": ([P, I64, P, P, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_float, ctypes.c_float, I, P], I),
         "oslmi_surrogate_set_routing": ([P, P, I], I),
+        "oslmi_surrogate_set_spot_check": ([P, ctypes.c_float], I),
         "oslmi_surrogate_infer": ([P, P, I64, P, P, P, P, P], I),
     }
     for name, (args, res) in sigs.items():

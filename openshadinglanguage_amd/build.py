@@ -24,6 +24,12 @@ HIPFLAGS = [
 ]
 
 
+# Per-source extra flags. The surrogate's MFMA kernels keep their
+# accumulators in VGPRs (the SiLU reads them there) instead of the
+# accumulation registers, which then hold the inference weights.
+EXTRA_FLAGS = {"oslmi_surrogate.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
@@ -36,7 +42,8 @@ def build(verbose=False, jobs=8):
     for src in sources():
         obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [HIPCC] + HIPFLAGS + inc + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + HIPFLAGS + EXTRA_FLAGS.get(os.path.basename(src), []) + inc + [
+            "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

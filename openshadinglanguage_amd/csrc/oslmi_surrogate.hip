@@ -16,14 +16,19 @@
 // the bf16 copies. All reductions run in a fixed order, so training is
 // deterministic.
 //
-// Inference: one fused kernel. Each wave keeps every weight fragment and
-// bias in registers and walks 16-query column tiles: a layer's MFMA
-// accumulators (lane: 4 consecutive features of one query) become the
-// next layer's B operand in place, because the packed weight fragments use
-// the matching order of the 64 hidden features along K. Queries that are
-// outside the training domain or in a routing cell flagged by calibration
-// are appended to a fallback list that the exact closure kernel then
-// evaluates.
+// Inference: one fused kernel. Each wave keeps every weight fragment in the
+// accumulation registers (the MFMA reads its A operand there), reads the
+// bias initialisers from LDS, and walks pairs of 16-query column tiles: a
+// layer's MFMA accumulators (lane: 4 consecutive features of one query, in
+// VGPRs) become the next layer's B operand in place, because the packed
+// weight fragments use the matching order of the 64 hidden features along
+// K. That fits a wave in 240 registers, two waves per SIMD. The SiLU costs
+// two transcendentals (exp2, rcp) and two VALU operations per unit, with
+// the -log2(e) scaling folded into the packed weights, and the output
+// epilogue is one exp2 (output normalisation folded into W5 and b5).
+// Queries that are outside the training domain or in a routing cell
+// flagged by calibration are appended to a fallback list that the exact
+// closure kernel then evaluates.
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -341,11 +346,19 @@ __host__ __device__ inline int hidden_k(int s, int g, int j)
 }
 
 // SiLU for inference with the hardware exp2 and reciprocal (v_exp_f32,
-// v_rcp_f32, about 1 ulp each): z / (1 + e^-z) = z * rcp(1 + 2^(-z log2 e)).
-// The result is rounded to bf16 next, far coarser than these errors.
-__device__ __forceinline__ float silu_fast(float z)
+// v_rcp_f32, about 1 ulp each), on pre-activations scaled by -log2(e):
+// the packed weights and biases of layers 1-4 carry that factor and W5 its
+// inverse, so with y = -z log2(e) the unit computes
+//   y * rcp(1 + 2^y) = -log2(e) * z / (1 + e^-z) = -log2(e) * SiLU(z)
+// (three VALU instructions and two transcendentals, no scaling multiply),
+// and every next layer sees its input scaled by the same -log2(e), which
+// W2..W4 (scaled by -log2(e) for their own output and by its inverse for
+// their input) and W5 absorb. The result is rounded to bf16 next, far
+// coarser than these errors.
+constexpr float kLog2e = 1.44269504088896341f;
+__device__ __forceinline__ float silu_fast(float y)
 {
-    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+    return y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y));
 }
 
 __device__ __forceinline__ bf16x8 act_pair(const f32x4& lo, const f32x4& hi)
@@ -476,17 +489,24 @@ __global__ __launch_bounds__(256) void k_route(Domain dom, const unsigned char* 
 }
 
 // The network over the queries listed in net[0 .. *netn).
+template<int TILES>
 __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __restrict__ frag,
                                                const f32x4* __restrict__ biasp,
                                                const int* __restrict__ net,
                                                const int* __restrict__ netn,
                                                const float* __restrict__ q, int64_t n,
-                                               float* __restrict__ out)
+                                               float* __restrict__ out, float min_weight,
+                                               int* __restrict__ fbidx, int* __restrict__ fbcount)
 {
+    // bias initialisers (MFMA C operands) in LDS, read as each layer starts;
+    // in registers they would cost 68 VGPRs per wave
+    __shared__ f32x4 s_b[NBIAS * 64];
+    for (int j = threadIdx.x; j < NBIAS * 64; j += blockDim.x)
+        s_b[j] = biasp[j];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     bf16x8 w1[4], w2[4][2], w3[4][2], w4[4][2], w5[2];
-    f32x4 c1[4], c2[4], c3[4], c4[4], c5;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         w1[t] = frag[t * 64 + lane];
@@ -496,17 +516,27 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
             w3[t][s] = frag[(12 + t * 2 + s) * 64 + lane];
             w4[t][s] = frag[(20 + t * 2 + s) * 64 + lane];
         }
-        c1[t] = biasp[t * 64 + lane];
-        c2[t] = biasp[(4 + t) * 64 + lane];
-        c3[t] = biasp[(8 + t) * 64 + lane];
-        c4[t] = biasp[(12 + t) * 64 + lane];
     }
     w5[0] = frag[28 * 64 + lane];
     w5[1] = frag[29 * 64 + lane];
-    c5    = biasp[16 * 64 + lane];
+    // the weights go to the accumulation registers, which the MFMA reads its
+    // A operand from directly; the VGPRs stay free for the accumulators and
+    // the SiLU work (no accumulator-to-VGPR copies)
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        asm volatile("" : "+a"(w1[t]));
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            asm volatile("" : "+a"(w2[t][s]));
+            asm volatile("" : "+a"(w3[t][s]));
+            asm volatile("" : "+a"(w4[t][s]));
+        }
+    }
+    asm volatile("" : "+a"(w5[0]));
+    asm volatile("" : "+a"(w5[1]));
 
     const int64_t cnt    = *netn;
-    const int64_t npairs = (cnt + 31) / 32;
+    const int64_t npairs = (cnt + 16 * TILES - 1) / (16 * TILES);
     const int64_t wave   = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     // Two 16-query tiles per iteration (independent MFMA/SiLU chains that the
@@ -515,18 +545,18 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
     // raw inputs are loaded while the current pair is computed.
     // loads are unconditional (clamped in-bounds addresses, results selected
     // afterwards) so that they issue back to back without branches
-    auto load_idx = [&](int64_t pair, int qi[2]) {
+    auto load_idx = [&](int64_t pair, int qi[TILES]) {
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int64_t e = (2 * pair + u) * 16 + r;
+        for (int u = 0; u < TILES; u++) {
+            const int64_t e = (TILES * pair + u) * 16 + r;
             const bool ok   = e < cnt;
             const int v     = net[ok ? e : 0];
             qi[u]           = v | (ok ? 0 : -1);  // -1 marks no query
         }
     };
-    auto load_x = [&](const int qi[2], float xr[2][8]) {
+    auto load_x = [&](const int qi[TILES], float xr[TILES][8]) {
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < TILES; u++) {
             const int64_t qs = qi[u] >= 0 ? qi[u] : 0;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -545,205 +575,110 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
         noff[j]     = k < NIN ? kc.nm.in_off[k < NIN ? k : 0] : 0.0f;
         nscl[j]     = k < NIN ? kc.nm.in_scale[k < NIN ? k : 0] : 0.0f;
     }
-    int qcur[2], qnext[2];
-    float xcur[2][8];
+    int qcur[TILES], qnext[TILES];
+    float xcur[TILES][8];
     load_idx(wave, qcur);
     load_x(qcur, xcur);
     load_idx(wave + nwaves, qnext);
     for (int64_t pair = wave; pair < npairs; pair += nwaves) {
-        float xnext[2][8];
-        int qnn[2];
+        // keeps the (loop-invariant) LDS bias reads inside the loop
+        asm volatile("" ::: "memory");
+        float xnext[TILES][8];
+        int qnn[TILES];
         load_x(qnext, xnext);
         load_idx(pair + 2 * nwaves, qnn);
-        f32x4 y2[2];
-        bf16x8 x[2];
+        f32x4 y2[TILES];
+        bf16x8 x[TILES];
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < TILES; u++)
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 // layer-1 operand: this lane group's 8 inputs (normalised)
                 const int k = 8 * g + j;
                 x[u][j] = (bf16)(k < NIN ? (xcur[u][j] - noff[j]) * nscl[j] : 0.0f);
             }
-        f32x4 a[2][4];
-        bf16x8 h0[2], h1[2];
+        f32x4 a[TILES][4];
+        bf16x8 h0[TILES], h1[TILES];
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < TILES; u++)
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x[u], c1[t], 0, 0, 0);
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x[u], s_b[t * 64 + lane], 0, 0, 0);
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < TILES; u++) {
             h0[u] = act_pair(a[u][0], a[u][1]);
             h1[u] = act_pair(a[u][2], a[u][3]);
         }
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < TILES; u++)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], h0[u], c2[t], 0, 0, 0);
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], h0[u], s_b[(4 + t) * 64 + lane], 0, 0, 0);
                 a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][1], h1[u], a[u][t], 0, 0, 0);
             }
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < TILES; u++) {
             h0[u] = act_pair(a[u][0], a[u][1]);
             h1[u] = act_pair(a[u][2], a[u][3]);
         }
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < TILES; u++)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], h0[u], c3[t], 0, 0, 0);
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], h0[u], s_b[(8 + t) * 64 + lane], 0, 0, 0);
                 a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][1], h1[u], a[u][t], 0, 0, 0);
             }
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < TILES; u++) {
             h0[u] = act_pair(a[u][0], a[u][1]);
             h1[u] = act_pair(a[u][2], a[u][3]);
         }
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < TILES; u++)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][0], h0[u], c4[t], 0, 0, 0);
+                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][0], h0[u], s_b[(12 + t) * 64 + lane], 0, 0, 0);
                 a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][1], h1[u], a[u][t], 0, 0, 0);
             }
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < TILES; u++) {
             h0[u] = act_pair(a[u][0], a[u][1]);
             h1[u] = act_pair(a[u][2], a[u][3]);
         }
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-            y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[0], h0[u], c5, 0, 0, 0);
+        for (int u = 0; u < TILES; u++) {
+            y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[0], h0[u], s_b[16 * 64 + lane], 0, 0, 0);
             y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[1], h1[u], y2[u], 0, 0, 0);
         }
 
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < TILES; u++) {
             // lanes of group 0 hold outputs 0..3 of their query
             const int qi = qcur[u];
             if (g == 0 && qi >= 0) {
+                float v[4];
 #pragma unroll
                 for (int o = 0; o < 4; o++)
-                    out[o * n + qi] = expf(y2[u][o] * kc.nm.out_sd[o] + kc.nm.out_mu[o]);
+                    v[o] = __builtin_amdgcn_exp2f(y2[u][o]);
+                // spot check: a predicted weight below the threshold has a
+                // large relative error (the log-space fit is absolute); such
+                // queries join the exact closure's list instead
+                if (fbidx && fminf(fminf(v[0], v[1]), v[2]) < min_weight) {
+                    fbidx[atomicAdd(fbcount, 1)] = qi;
+                } else {
+#pragma unroll
+                    for (int o = 0; o < 4; o++)
+                        out[o * n + qi] = v[o];
+                }
             }
         }
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < TILES; u++) {
             qcur[u]  = qnext[u];
             qnext[u] = qnn[u];
 #pragma unroll
             for (int j = 0; j < 8; j++)
                 xcur[u][j] = xnext[u][j];
-        }
-    }
-}
-
-// The network over net[0 .. *netn) with the weights in LDS: every
-// workgroup (8 waves) copies the packed fragments and bias initialisers
-// (48 KB) to LDS once; MFMA A operands are then read from LDS as each
-// layer needs them. With the weights out of the register file a wave
-// needs about a third of the registers, so four waves share a SIMD and one
-// wave's SiLU work overlaps another's MFMAs. TILES independent 16-query
-// tiles per iteration share every weight read.
-constexpr int INFER_BLOCK = 512;
-template<int TILES>
-__global__ __launch_bounds__(INFER_BLOCK, 2) void k_infer_lds(InferConsts kc,
-                                                              const bf16x8* __restrict__ frag,
-                                                              const f32x4* __restrict__ biasp,
-                                                              const int* __restrict__ net,
-                                                              const int* __restrict__ netn,
-                                                              const float* __restrict__ q,
-                                                              int64_t n, float* __restrict__ out)
-{
-    __shared__ bf16x8 s_w[NFRAG * 64];
-    __shared__ f32x4 s_b[NBIAS * 64];
-    for (int j = threadIdx.x; j < NFRAG * 64; j += INFER_BLOCK)
-        s_w[j] = frag[j];
-    for (int j = threadIdx.x; j < NBIAS * 64; j += INFER_BLOCK)
-        s_b[j] = biasp[j];
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int g = lane >> 4, r = lane & 15;
-    const int64_t cnt    = *netn;
-    const int64_t ngroup = (cnt + 16 * TILES - 1) / (16 * TILES);
-    const int64_t wave   = ((int64_t)blockIdx.x * INFER_BLOCK + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * INFER_BLOCK) >> 6;
-    float noff[8], nscl[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int k = 8 * g + j;
-        noff[j]     = k < NIN ? kc.nm.in_off[k < NIN ? k : 0] : 0.0f;
-        nscl[j]     = k < NIN ? kc.nm.in_scale[k < NIN ? k : 0] : 0.0f;
-    }
-    for (int64_t grp = wave; grp < ngroup; grp += nwaves) {
-        // keeps the (loop-invariant) LDS weight reads inside the loop: held
-        // across iterations they would need the registers this kernel frees
-        asm volatile("" ::: "memory");
-        int qi[TILES];
-        bf16x8 x[TILES];
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            const int64_t e = (grp * TILES + u) * 16 + r;
-            const bool ok   = e < cnt;
-            qi[u]           = ok ? net[e] : -1;
-            const int64_t qs = ok ? qi[u] : 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int k   = 8 * g + j;
-                const int kc_ = k < NIN ? k : NIN - 1;
-                const float v = q[kc_ * n + qs];
-                x[u][j]       = (bf16)(k < NIN && ok ? (v - noff[j]) * nscl[j] : 0.0f);
-            }
-        }
-        f32x4 a[TILES][4];
-        bf16x8 h0[TILES], h1[TILES];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const bf16x8 w  = s_w[t * 64 + lane];
-            const f32x4 c   = s_b[t * 64 + lane];
-#pragma unroll
-            for (int u = 0; u < TILES; u++)
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, x[u], c, 0, 0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            h0[u] = act_pair(a[u][0], a[u][1]);
-            h1[u] = act_pair(a[u][2], a[u][3]);
-        }
-#pragma unroll
-        for (int l = 0; l < 3; l++) {
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const bf16x8 w0 = s_w[(4 + l * 8 + t * 2) * 64 + lane];
-                const bf16x8 w1 = s_w[(4 + l * 8 + t * 2 + 1) * 64 + lane];
-                const f32x4 c   = s_b[((l + 1) * 4 + t) * 64 + lane];
-#pragma unroll
-                for (int u = 0; u < TILES; u++) {
-                    a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, h0[u], c, 0, 0, 0);
-                    a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, h1[u], a[u][t], 0, 0, 0);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < TILES; u++) {
-                h0[u] = act_pair(a[u][0], a[u][1]);
-                h1[u] = act_pair(a[u][2], a[u][3]);
-            }
-        }
-        const bf16x8 w50 = s_w[28 * 64 + lane];
-        const bf16x8 w51 = s_w[29 * 64 + lane];
-        const f32x4 c5   = s_b[16 * 64 + lane];
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            f32x4 y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w50, h0[u], c5, 0, 0, 0);
-            y       = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w51, h1[u], y, 0, 0, 0);
-            // lanes of group 0 hold outputs 0..3 of their query
-            if (g == 0 && qi[u] >= 0) {
-#pragma unroll
-                for (int o = 0; o < 4; o++)
-                    out[o * n + qi[u]] = expf(y[o] * kc.nm.out_sd[o] + kc.nm.out_mu[o]);
-            }
         }
     }
 }
@@ -776,6 +711,7 @@ struct Context {
     f32x4* biasp = nullptr;
     unsigned char* cells = nullptr;
     int* fbn = nullptr;     // network-list count
+    float min_weight = 0.0f;  // spot-check threshold on predicted weights
     int* netlist = nullptr;  // network-list indices (grown on demand)
     int64_t netcap = 0;
     bool packed = false;
@@ -882,6 +818,14 @@ int fwd_bwd(Context* c, int64_t B64, const float* q, const float* t, int64_t ldq
 }
 
 // Packs the inference fragments and bias initialisers from host parameters.
+// Scale of W5's output row o in the packed weights: the network's output
+// y_o (normalised log target) becomes log2 of the answer,
+//   log2(exp(y_o sd_o + mu_o)) = (y_o sd_o + mu_o) log2(e),
+// and the layer's input arrives scaled by -log2(e) (see silu_fast), so the
+// row carries -sd_o and the bias (b_o sd_o + mu_o) log2(e); the kernel's
+// epilogue is then a single exp2.
+float out_row_scale(const Norm& nm, int o) { return o < NOUT ? -nm.out_sd[o] : 0.0f; }
+
 int pack_inference(Context* c)
 {
     const std::vector<float>& P = c->host_params;
@@ -892,8 +836,8 @@ int pack_inference(Context* c)
     for (int t = 0; t < 4; t++, f++)
         for (int lane = 0; lane < 64; lane++)
             for (int j = 0; j < 8; j++)
-                frag[((size_t)f * 64 + lane) * 8 + j] = (bf16)W(1, 16 * t + (lane & 15),
-                                                                8 * (lane >> 4) + j);
+                frag[((size_t)f * 64 + lane) * 8 + j] =
+                    (bf16)(-kLog2e * W(1, 16 * t + (lane & 15), 8 * (lane >> 4) + j));
     for (int l = 2; l <= 4; l++)
         for (int t = 0; t < 4; t++)
             for (int s = 0; s < 2; s++, f++)
@@ -905,16 +849,22 @@ int pack_inference(Context* c)
         for (int lane = 0; lane < 64; lane++)
             for (int j = 0; j < 8; j++)
                 frag[((size_t)f * 64 + lane) * 8 + j] =
-                    (bf16)W(5, lane & 15, hidden_k(s, lane >> 4, j));
+                    (bf16)(W(5, lane & 15, hidden_k(s, lane >> 4, j)) * out_row_scale(c->nm, lane & 15));
     int b = 0;
     for (int l = 1; l <= 4; l++)
         for (int t = 0; t < 4; t++, b++)
             for (int lane = 0; lane < 64; lane++)
                 for (int j = 0; j < 4; j++)
-                    bias[((size_t)b * 64 + lane) * 4 + j] = P[OFF_B[l] + 16 * t + 4 * (lane >> 4) + j];
+                    bias[((size_t)b * 64 + lane) * 4 + j] =
+                        -kLog2e * P[OFF_B[l] + 16 * t + 4 * (lane >> 4) + j];
     for (int lane = 0; lane < 64; lane++)
-        for (int j = 0; j < 4; j++)
-            bias[((size_t)b * 64 + lane) * 4 + j] = P[OFF_B[5] + 4 * (lane >> 4) + j];
+        for (int j = 0; j < 4; j++) {
+            const int o = 4 * (lane >> 4) + j;
+            float v     = P[OFF_B[5] + o];
+            if (o < NOUT)
+                v = (v * c->nm.out_sd[o] + c->nm.out_mu[o]) * kLog2e;
+            bias[((size_t)b * 64 + lane) * 4 + j] = v;
+        }
     if (int rc = check_hip(hipMemcpy(c->frag, frag.data(), frag.size() * sizeof(bf16),
                                      hipMemcpyHostToDevice), "hipMemcpy(fragments)"))
         return rc;
@@ -993,6 +943,7 @@ extern "C" int oslmi_surrogate_set_normalization(oslmi_surrogate* s, const float
     memcpy(s->c.nm.out_sd, out_std, sizeof(float) * NOUT);
     memcpy(s->c.dom.lo, domain_lo, sizeof(float) * 9);
     memcpy(s->c.dom.hi, domain_hi, sizeof(float) * 9);
+    s->c.packed = false;  // the packed output layer carries the output scaling
     return OSLMI_OK;
 }
 
@@ -1091,6 +1042,14 @@ extern "C" int oslmi_surrogate_set_routing(oslmi_surrogate* s, const unsigned ch
                      "hipMemcpy(routing)");
 }
 
+extern "C" int oslmi_surrogate_set_spot_check(oslmi_surrogate* s, float min_weight)
+{
+    if (!s || !(min_weight >= 0.0f))
+        return set_error(OSLMI_ERR_INVALID, "oslmi_surrogate_set_spot_check: need min_weight >= 0");
+    s->c.min_weight = min_weight;
+    return OSLMI_OK;
+}
+
 static unsigned infer_grid(int64_t n)
 {
     int64_t waves = (n + 15) / 16;
@@ -1138,20 +1097,10 @@ extern "C" int oslmi_surrogate_infer(oslmi_surrogate* s, const oslmi_layered_mat
     hipLaunchKernelGGL(k_route, dim3((unsigned)rblocks), dim3(256), 0, st, s->c.dom, s->c.cells,
                        mode, d_query, n, d_out, s->c.netlist, s->c.fbn, d_fallback_index,
                        d_fallback_count);
-    const int variant = getenv("OSLMI_INFER_KERNEL") ? atoi(getenv("OSLMI_INFER_KERNEL")) : 0;
-    if (variant == 0) {
-        hipLaunchKernelGGL(k_infer, dim3(infer_grid(n)), dim3(256), 0, st, kc, s->c.frag,
-                           s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out);
-    } else {
-        int64_t blocks = (n + 16 * variant * 8 - 1) / (16 * variant * 8);
-        blocks         = std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 2));
-        if (variant == 4)
-            hipLaunchKernelGGL(k_infer_lds<4>, dim3((unsigned)blocks), dim3(INFER_BLOCK), 0, st, kc,
-                               s->c.frag, s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out);
-        else
-            hipLaunchKernelGGL(k_infer_lds<2>, dim3((unsigned)blocks), dim3(INFER_BLOCK), 0, st, kc,
-                               s->c.frag, s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out);
-    }
+    hipLaunchKernelGGL(k_infer<2>, dim3(infer_grid(n)), dim3(256), 0, st, kc, s->c.frag,
+                       s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out, s->c.min_weight,
+                       mode == 1 ? d_fallback_index : nullptr,
+                       mode == 1 ? d_fallback_count : nullptr);
     if (int rc = check_launch("oslmi_surrogate_infer"))
         return rc;
     if (mode == 1)

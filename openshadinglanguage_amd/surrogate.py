@@ -167,6 +167,9 @@ class Surrogate:
         check(self.L.oslmi_surrogate_set_routing(self.h, cells.ctypes.data_as(ctypes.c_void_p),
                                                  cells.size))
 
+    def set_spot_check(self, min_weight):
+        check(self.L.oslmi_surrogate_set_spot_check(self.h, float(min_weight)))
+
     def infer(self, queries, out, material=None, fb_index=None, fb_count=None, stream=None):
         import torch
         stream = stream or torch.cuda.current_stream()
@@ -199,18 +202,40 @@ def relative_errors(pred, exact):
     return ew, ep
 
 
-def calibrate(pred, exact, q, budget=0.01):
-    """Routing table: cells whose mean relative error (weight or pdf) on the
-    calibration queries exceeds `budget` go to the exact closure."""
+def calibrate(pred, exact, q, budget=0.01, margin=0.8, min_weight=0.0):
+    """Routing table from held-out calibration queries: routing cells go to
+    the exact closure, the cell with the largest mean network error first,
+    until the mean relative error over all calibration queries (answers of
+    the exact closure, including the spot-checked ones whose predicted
+    weight is below min_weight, count as error-free) is at most
+    margin * budget for both the weight and the pdf. Returns (table,
+    per-cell stats, predicted mean errors (weight, pdf), predicted fallback
+    fraction)."""
     ew, ep = relative_errors(pred, exact)
-    err = np.maximum(ew, ep)
+    spot = np.asarray(pred, np.float64)[:3].min(0) < min_weight
+    ew = np.where(spot, 0.0, ew)
+    ep = np.where(spot, 0.0, ep)
     cells = route_cells(q)
+    n = float(len(ew))
     table = np.zeros(NCELL, np.uint8)
     stats = []
+    sw = np.zeros(NCELL)
+    sp = np.zeros(NCELL)
+    cnt = np.zeros(NCELL)
     for c in range(NCELL):
         sel = cells == c
-        if sel.any():
-            e = float(err[sel].mean())
-            table[c] = 1 if e > budget else 0
-            stats.append((c, int(sel.sum()), e))
-    return table, stats
+        cnt[c] = sel.sum()
+        if cnt[c]:
+            sw[c] = ew[sel].sum()
+            sp[c] = ep[sel].sum()
+            stats.append((c, int(cnt[c]), float(max(sw[c], sp[c]) / cnt[c])))
+    mw, mp = sw.sum() / n, sp.sum() / n
+    order = sorted(range(NCELL), key=lambda c: -(max(sw[c], sp[c]) / cnt[c] if cnt[c] else 0))
+    for c in order:
+        if max(mw, mp) <= margin * budget or not cnt[c]:
+            break
+        table[c] = 1
+        mw -= sw[c] / n
+        mp -= sp[c] / n
+    routed = np.isin(cells, np.nonzero(table)[0])
+    return table, stats, (float(mw), float(mp)), float((routed | spot).mean())

@@ -158,3 +158,38 @@ def test_full_routing_reproduces_exact(dev):
     s.infer(q, out, material=m, fb_index=idx, fb_count=cnt)
     torch.cuda.synchronize()
     assert int(cnt.item()) == 0
+
+
+def test_spot_check_sends_low_predictions_to_exact(dev):
+    """With the spot-check threshold above every prediction, all network
+    queries join the fallback list and the answer equals the exact closure;
+    with a threshold between, exactly the queries predicted below it do."""
+    from openshadinglanguage_amd import check, lib
+    n = 50000
+    q = torch.from_numpy(W.surrogate_queries(n, seed=26)).to(dev)
+    m = W.layered_material()
+    exact = torch.empty((4, n), device=dev)
+    check(lib().oslmi_layered_eval_local(ctypes.byref(m), n, S._p(q), S._p(exact), None))
+    s = S.Surrogate(max_batch=2048)
+    s.set_routing(np.zeros(S.NCELL, np.uint8))
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    net = torch.full((4, n), -1.0, device=dev)
+    s.infer(q, net)  # network answers for every upper-hemisphere query
+    s.set_spot_check(1e30)
+    out = torch.full((4, n), -1.0, device=dev)
+    s.infer(q, out, material=m, fb_index=idx, fb_count=cnt)
+    torch.cuda.synchronize()
+    assert torch.equal(out, exact)
+    upper = (q[4] > 0).logical_and(q[2] > 0)
+    assert int(cnt.item()) == int(upper.sum())
+    thr = float(net[:3, upper].min(0).values.median())
+    s.set_spot_check(thr)
+    s.infer(q, out, material=m, fb_index=idx, fb_count=cnt)
+    torch.cuda.synchronize()
+    low = upper.logical_and(net[:3].min(0).values < thr)
+    assert int(cnt.item()) == int(low.sum())
+    assert torch.equal(out[:, low], exact[:, low])
+    assert torch.equal(out[:, upper & ~low], net[:, upper & ~low])
+    with pytest.raises(Exception):
+        s.set_spot_check(-1.0)
