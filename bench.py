@@ -43,7 +43,7 @@ WORKLOADS = {
 # (profiles/*_pmc.txt), in bytes; None when not measured.
 # FETCH_SIZE reads half the bytes of a plain 256 MiB copy on this GPU (x2
 # scale), WRITE_SIZE reads them exactly (x1); profiles/layered3_pmc.txt.
-MEASURED_TRAFFIC = {"layered3": 2 * 24816.98 * 1024 + 16384.0 * 1024, "fbm_cell": None}
+MEASURED_TRAFFIC = {"layered3": 2 * 24797.7 * 1024 + 16384.0 * 1024, "fbm_cell": None}
 
 
 def _ops_per_point(name):
