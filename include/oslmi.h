@@ -141,7 +141,8 @@ typedef struct oslmi_batch_item {
  * cost per tile, most expensive first; both keep each group's tiles
  * together), streams the batch's kernels are spread over (1..4; more than
  * one lets the kernels overlap, forked from and joined back into the
- * caller's stream). */
+ * caller's stream; eager launches only: a captured graph records the
+ * kernels on the capturing stream). */
 typedef struct oslmi_batch_config {
     int block, ppt, blocks_per_cu, bb_lds, order, streams;
 } oslmi_batch_config;
@@ -169,12 +170,15 @@ int oslmi_batch_launch(oslmi_batch* b, void* stream);
 int oslmi_batch_launch_per_group(oslmi_batch* b, void* stream);
 /* Captures oslmi_batch_launch into a hipGraph for replay on `stream`. */
 int oslmi_batch_capture(oslmi_batch* b, void* stream);
-/* Times `reps` launches of every candidate shape (block x ppt x
- * blocks_per_cu x bb_lds x order) and keeps the fastest; writes it to
- * *best and its mean time in ms to *best_ms. Runs the kernel, so the
- * bound outputs are overwritten with the (identical) results. */
-int oslmi_batch_autotune(oslmi_batch* b, void* stream, int reps, oslmi_batch_config* best,
-                         double* best_ms);
+/* Times `reps` launches of candidate shapes (block x ppt x blocks_per_cu,
+ * then bb_lds x order x streams) and keeps the fastest; writes it to *best
+ * and its mean time in ms to *best_ms. With graph != 0 every candidate is
+ * timed as a captured graph replayed on `stream` (a created stream), and
+ * the winner stays captured for oslmi_batch_launch on that stream. Runs the
+ * kernels, so the bound outputs are overwritten with the (identical)
+ * results. */
+int oslmi_batch_autotune(oslmi_batch* b, void* stream, int reps, int graph,
+                         oslmi_batch_config* best, double* best_ms);
 
 /* ---- Prebuilt group lowerings -----------------------------------------
  * The two benchmark groups compiled ahead of time (OSL sources in

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cstring>
 #include <numeric>
@@ -83,6 +84,7 @@ struct Batch {
     hipEvent_t fork_ev = nullptr;
     std::vector<hipEvent_t> join_ev;
     // captured launches
+    bool in_capture          = false;
     hipGraph_t graph         = nullptr;
     hipGraphExec_t gexec     = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -244,12 +246,15 @@ struct Batch {
 
     // Every part's kernel over its tiles. With cfg.streams > 1 the kernels
     // are dealt round-robin to side streams that wait for the caller's
-    // stream and are waited for by it (a fork/join that stream capture
-    // records as parallel graph branches).
+    // stream and are waited for by it. Under graph capture (in_capture) the
+    // kernels go to the capturing stream one after another: a captured
+    // fork/join replayed slower than eager launches, and a run that tuned
+    // many such graphs ended in a host crash whose cause was not found, so
+    // captured graphs keep to one stream.
     int launch_all(hipStream_t stream)
     {
         const int ni = (int)hargs.size();
-        const int ns = std::min<int>(cfg.streams, (int)parts.size());
+        const int ns = in_capture ? 1 : std::min<int>(cfg.streams, (int)parts.size());
         if (ns <= 1) {
             for (size_t q = 0; q < parts.size(); ++q)
                 if (int rc = launch_range(parts[q], parts[q].first_tile, parts[q].ntiles, ni + (int)q, stream))
@@ -281,6 +286,32 @@ struct Batch {
         return launch_range(parts[part_of(k)], item_first[k], item_count[k], k, stream);
     }
 
+    // Records launch_all on `st` into a graph (replaces any previous one).
+    int capture(hipStream_t st)
+    {
+        drop_graph();
+        if (int rc = check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
+                               "hipStreamBeginCapture"))
+            return rc;
+        in_capture   = true;
+        int lrc      = launch_all(st);
+        in_capture   = false;
+        hipGraph_t g = nullptr;
+        int erc      = check_hip(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
+        if (lrc || erc) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return lrc ? lrc : erc;
+        }
+        graph = g;
+        if (int rc = check_hip(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+            drop_graph();
+            return rc;
+        }
+        graph_stream = st;
+        return OSLMI_OK;
+    }
+
     int launch_per_group(hipStream_t stream)
     {
         for (int k = 0; k < (int)hargs.size(); ++k)
@@ -305,6 +336,45 @@ int validate(const oslmi_batch_config& c)
     if (c.streams < 1 || c.streams > 4)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch: streams must be in 1..4");
     return OSLMI_OK;
+}
+
+// fn(0 .. n-1) on up to 16 host threads
+template <class F>
+void parallel_for(int n, F fn)
+{
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<int> next { 0 };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::min(hw, n); ++t)
+        pool.emplace_back([&] {
+            for (int k; (k = next.fetch_add(1)) < n;)
+                fn(k);
+        });
+    for (std::thread& t : pool)
+        t.join();
+}
+
+// VGPR + AGPR count of the (single) kernel in a gfx950 code object, read
+// from its metadata note (msgpack map entries ".vgpr_count" and
+// ".agpr_count"); 0 when absent.
+int code_object_registers(const std::vector<char>& code)
+{
+    auto field = [&](const char* key) -> int {
+        const size_t kl = strlen(key);
+        for (size_t i = 0; i + kl + 3 < code.size(); ++i) {
+            if (memcmp(code.data() + i, key, kl) != 0)
+                continue;
+            const unsigned char* v = (const unsigned char*)code.data() + i + kl;
+            if (v[0] < 0x80)
+                return v[0];  // positive fixint
+            if (v[0] == 0xcc)
+                return v[1];  // uint8
+            if (v[0] == 0xcd)
+                return (v[1] << 8) | v[2];  // uint16, big endian
+        }
+        return 0;
+    };
+    return field(".vgpr_count") + field(".agpr_count");
 }
 
 // mean ms of `reps` calls of f on stream (after one untimed call)
@@ -376,13 +446,51 @@ oslmi_batch* oslmi_batch_create(oslmi_shadingsys* s, int ngroups, oslmi_group* c
             }
         b.groups.push_back(groups[k]->g);
     }
+    // Register need of every group: its own kernel's VGPR + AGPR count
+    // (compiled through the code-object cache). Groups of similar need
+    // share a kernel, whose register budget is the maximum over its groups,
+    // so light groups keep their occupancy.
+    std::vector<int> regs(ngroups, 0);
+    {
+        std::vector<int> rcs(ngroups, OSLMI_OK);
+        std::vector<std::string> errs(ngroups);
+        parallel_for(ngroups, [&](int k) {
+            Group& g = *b.groups[k];
+            std::string src, err;
+            std::vector<char> code;
+            bool ok;
+            {
+                std::lock_guard<std::mutex> lock(g.mu);
+                ok = emit_group(s->ss, g, src, err);
+            }
+            if (!ok) {
+                rcs[k]  = OSLMI_ERR_COMPILE;
+                errs[k] = "emit " + g.name + ": " + err;
+                return;
+            }
+            if ((rcs[k] = build_code_object(s->ss, src, g.name, code, nullptr)) != OSLMI_OK) {
+                errs[k] = oslmi_last_error();
+                return;
+            }
+            regs[k] = code_object_registers(code);
+        });
+        for (int k = 0; k < ngroups; ++k)
+            if (rcs[k]) {
+                set_error(rcs[k], "oslmi_batch_create: " + errs[k]);
+                delete h;
+                return nullptr;
+            }
+    }
+    std::vector<int> order(ngroups);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return regs[a] < regs[c]; });
     // emit (the emitter records each group's outputs, under its lock)
     for (int first = 0; first < ngroups; first += per) {
         Part p;
         std::vector<Group*> gs;
-        for (int k = first; k < std::min(ngroups, first + per); ++k) {
-            p.items.push_back(k);
-            gs.push_back(b.groups[k]);
+        for (int j = first; j < std::min(ngroups, first + per); ++j) {
+            p.items.push_back(order[j]);
+            gs.push_back(b.groups[order[j]]);
         }
         std::string err;
         bool ok;
@@ -551,31 +659,14 @@ int oslmi_batch_capture(oslmi_batch* h, void* stream)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch_capture: batch not bound");
     if (!stream)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch_capture: capture needs a created stream, not the null stream");
-    Batch& b       = h->b;
-    hipStream_t st = (hipStream_t)stream;
-    b.drop_graph();
-    if (int rc = check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
-        return rc;
-    int lrc      = b.launch_all(st);
-    hipGraph_t g = nullptr;
-    int erc      = check_hip(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
-    if (lrc || erc) {
-        if (g)
-            (void)hipGraphDestroy(g);
-        return lrc ? lrc : erc;
-    }
-    b.graph = g;
-    if (int rc = check_hip(hipGraphInstantiate(&b.gexec, g, nullptr, nullptr, 0), "hipGraphInstantiate")) {
-        b.drop_graph();
-        return rc;
-    }
-    b.graph_stream = st;
-    return OSLMI_OK;
+    return h->b.capture((hipStream_t)stream);
 }
 
-int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_config* best,
-                         double* best_ms)
+int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, int graph,
+                         oslmi_batch_config* best, double* best_ms)
 {
+    if (graph && !stream)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_autotune: graph tuning needs a created stream");
     if (!h || !h->b.bound)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch_autotune: batch not bound");
     Batch& b       = h->b;
@@ -607,7 +698,8 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_con
     // two stages: the launch geometry (block, ppt, blocks_per_cu) with the
     // LDS table, cost order and all side streams; then those three
     // switches with the winning geometry
-    const int max_streams = std::min<int>(4, (int)b.parts.size());
+    // (captured graphs keep to one stream, see launch_all)
+    const int max_streams = graph ? 1 : std::min<int>(4, (int)b.parts.size());
     oslmi_batch_config win{ 256, 1, 8, 1, 1, max_streams };
     double win_ms = 1e30;
     auto trial = [&](const oslmi_batch_config& c) -> int {
@@ -615,8 +707,18 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_con
         if (int rc = b.build_tiles())
             return rc;
         double ms = 0.0;
-        if (int rc = time_ms(st, reps, [&] { return b.launch_all(st); }, ms))
+        if (graph) {
+            // timed the way it will run: as a replayed graph
+            if (int rc = b.capture(st))
+                return rc;
+            if (int rc = time_ms(st, reps, [&] {
+                    return check_hip(hipGraphLaunch(b.gexec, st), "hipGraphLaunch");
+                }, ms))
+                return rc;
+            b.drop_graph();
+        } else if (int rc = time_ms(st, reps, [&] { return b.launch_all(st); }, ms)) {
             return rc;
+        }
         if (ms < win_ms) {
             win_ms = ms;
             win    = c;
@@ -647,6 +749,9 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, oslmi_batch_con
     b.cfg = win;
     if (int rc = b.build_tiles())
         return rc;
+    if (graph)
+        if (int rc = b.capture(st))
+            return rc;
     if (best)
         *best = win;
     if (best_ms)

@@ -293,9 +293,11 @@ class GroupBatch:
     def capture(self, stream):
         check(lib().oslmi_batch_capture(self.h, self._stream(stream)))
 
-    def autotune(self, stream=None, reps=3):
+    def autotune(self, stream=None, reps=3, graph=False):
+        """Tunes the launch shape; with graph=True every candidate is timed
+        as a replayed hipGraph on `stream` and the winner stays captured."""
         c = BatchConfig()
         ms = ctypes.c_double()
-        check(lib().oslmi_batch_autotune(self.h, self._stream(stream), reps, ctypes.byref(c),
-                                         ctypes.byref(ms)))
+        check(lib().oslmi_batch_autotune(self.h, self._stream(stream), reps, 1 if graph else 0,
+                                         ctypes.byref(c), ctypes.byref(ms)))
         return c.as_dict(), ms.value

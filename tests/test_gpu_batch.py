@@ -188,6 +188,20 @@ def test_autotune_keeps_results(setup):
     for k, (r, g) in enumerate(zip(ref, _cpu(outs))):
         np.testing.assert_array_equal(r[0], g[0], err_msg="group %d" % k)
         np.testing.assert_array_equal(r[1], g[1], err_msg="group %d" % k)
+    # tuned as captured graphs: the winner is replayed on the tuning stream
+    s = torch.cuda.Stream()
+    best, ms = b.autotune(stream=s, reps=1, graph=True)
+    assert ms > 0 and b.config().as_dict() == best
+    for rep in range(2):
+        for o, e in outs:
+            o.fill_(-7.0)
+            e.fill_(-7.0)
+        torch.cuda.synchronize()
+        b.launch(stream=s)
+        s.synchronize()
+        for k, (r, g) in enumerate(zip(ref, _cpu(outs))):
+            np.testing.assert_array_equal(r[0], g[0], err_msg="graph group %d" % k)
+            np.testing.assert_array_equal(r[1], g[1], err_msg="graph group %d" % k)
 
 
 def test_empty_items_are_skipped(setup):
