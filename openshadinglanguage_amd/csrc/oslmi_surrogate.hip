@@ -393,26 +393,31 @@ __device__ __forceinline__ int route_cell(float lx, float lz, float vx, float vz
 // otherwise -> fallback list (mode 1) or NaN outputs (mode 0).
 // class of query i: 0 exact zero, 1 network, 2 fallback (mode 1) or not
 // answered (mode 0)
+// All 13 planes are loaded unconditionally (one batch of independent loads
+// per query, no branch between them) and the class is formed with selects.
 __device__ __forceinline__ int route_class(const Domain& dom, const unsigned char* cells, int mode,
                                            const float* __restrict__ q, int64_t n, int64_t i)
 {
-    const float lz = q[2 * n + i], vz = q[4 * n + i];
-    if (vz > 0.0f && lz <= 0.0f)
-        return 0;
+    float v[14];
+#pragma unroll
+    for (int k = 0; k < 14; k++)
+        v[k] = k == 1 ? 0.0f : q[k * n + i];
+    const float lz = v[2], vz = v[4];
     bool inside = vz > 0.0f;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const float p = q[(5 + k) * n + i];
-        inside = inside && p >= dom.lo[k] && p <= dom.hi[k];
-    }
-    if (inside && mode == 1 && cells[route_cell(q[i], lz, q[3 * n + i], vz, q[6 * n + i])])
+    for (int k = 0; k < 9; k++)
+        inside = inside && v[5 + k] >= dom.lo[k] && v[5 + k] <= dom.hi[k];
+    const int cell = route_cell(v[0], lz, v[3], vz, v[6]);
+    if (inside && mode == 1 && cells[cell])
         inside = false;
+    if (vz > 0.0f && lz <= 0.0f)
+        return 0;
     return inside ? 1 : 2;
 }
 
-constexpr int ROUTE_CHUNK = 4096;  // queries per workgroup (16 rounds of 256)
+constexpr int ROUTE_CHUNK = 1024;  // queries per workgroup (4 rounds of 256)
 
-// Order-preserving compaction: workgroup b owns queries [b*4096, (b+1)*4096).
+// Order-preserving compaction: workgroup b owns queries [b*1024, (b+1)*1024).
 // Pass 1 classifies them (one read of the query planes; classes kept in
 // LDS) and counts network and fallback queries, one atomic per list
 // reserves the workgroup's slots, pass 2 writes the indices in query order
@@ -429,6 +434,7 @@ __global__ __launch_bounds__(256) void k_route(Domain dom, const unsigned char* 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * ROUTE_CHUNK;
     int mine[2] = { 0, 0 };
+#pragma unroll
     for (int r = 0; r < ROUTE_CHUNK / 256; r++) {
         const int64_t i = c0 + r * 256 + threadIdx.x;
         int c = -1;
