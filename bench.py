@@ -381,7 +381,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-group batch measurement")
     ap.add_argument("--train-steps", type=int, default=60000)
-    ap.add_argument("--cpu-passes", type=int, default=2)
+    # 6 passes over the full workload: about 1 s wall, 15-20 s of CPU work
+    # on 16 host threads
+    ap.add_argument("--cpu-passes", type=int, default=6)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
