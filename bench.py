@@ -151,6 +151,51 @@ def bench_closure(steps, warmup, cpu, cpu_passes):
     return res
 
 
+def bench_small(steps, warmup):
+    """The single-layer fbm_cell group over 256x256 points: 65,536 points
+    are a few microseconds of work, so the launch cost dominates a single
+    launch; 100 launches captured in one hipGraph and replayed amortise it."""
+    import torch
+    from openshadinglanguage_amd import shadingsys as S
+    from openshadinglanguage_amd import workloads as W
+    dev = torch.device("cuda:0")
+    inp = W.fbm_cell_inputs(256, seed=1)
+    n = inp["P"].shape[1]
+    P = torch.from_numpy(inp["P"]).to(dev)
+    out = torch.empty_like(P)
+    ss, grp = S.build_fbm_cell()
+    ss.jit_group(grp)
+    side = torch.cuda.Stream()
+    step = ss.bind(grp, n, {"P": P}, outputs=out, stream=side)
+    eager_ms, _ = _timed(step, side, steps, warmup)
+    torch.cuda.synchronize()
+    ref = out.clone()
+    reps = 100
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        for _ in range(reps):
+            step()
+    torch.cuda.synchronize()
+    out.fill_(-1.0)
+    # replay() launches the graph on the current stream
+    graph_ms, _ = _timed(g.replay, torch.cuda.current_stream(), max(3, steps // 10), 1)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(out, ref))
+    per_launch = graph_ms / reps
+    ops = _ops_per_point("fbm_cell")
+    achieved = ops * n / (per_launch / 1e3)
+    return {"workload": "fbm_cell: 4-octave perlin fBm + 0.25 cellnoise(3P) -> smoothstep/mix, "
+                        "256x256 points, no derivs",
+            "points": n, "eager_ms_per_launch": round(eager_ms, 5),
+            "graph_ms_per_launch": round(per_launch, 5), "launches_per_graph": reps,
+            "Mpts_per_s": round(n / (per_launch / 1e3) / 1e6, 1),
+            "graph_matches_eager": same,
+            "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                         "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
+                         "frac": round(achieved / VALU_PEAK_OPS, 4), "traffic": None,
+                         "ops_per_point": round(ops, 1), "kernel_ms": round(per_launch, 5)}}
+
+
 def bench_batch(steps, warmup, cpu, autotune_reps=2):
     """The 64 generated groups (seed 3, log-uniform point counts): one
     persistent batch launch (autotuned, replayed from a hipGraph) against
@@ -380,6 +425,8 @@ def main():
                     help="skip the surrogate training + measurement")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-group batch measurement")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 256x256 single-layer group (graph-replayed) measurement")
     ap.add_argument("--train-steps", type=int, default=60000)
     # 6 passes over the full workload: about 1 s wall, 15-20 s of CPU work
     # on 16 host threads
@@ -497,6 +544,8 @@ def main():
         cpu_inp = inp if name == "fbm_cell" else inp
         line["cpu_baseline"] = cpu_baseline(name, cpu_inp, nthreads, args.cpu_passes)
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    if not args.no_small and name != "fbm_cell":
+        line["small"] = bench_small(max(20, args.steps), args.warmup)
     if not args.no_batch:
         line["batch"] = bench_batch(max(5, args.steps // 2), args.warmup, not args.no_cpu)
     if not args.no_closure:
