@@ -215,3 +215,32 @@ def test_empty_items_are_skipped(setup):
     torch.cuda.synchronize()
     for k in range(0, NG, 3):
         assert (outs[k][0] != -7.0).all()
+
+
+def test_second_group_set_matches_interpreter():
+    """16 more generated groups (seed 11), each through its own emitted
+    kernel (oslmi_execute_batch), against the oracle's interpreter."""
+    from openshadinglanguage_amd import check, lib
+    from openshadinglanguage_amd import shadingsys as S
+    check(lib().oslmi_init())
+    dev = torch.device("cuda:0")
+    specs, _ = G.make_groups(16, 11)
+    ss = S.ShadingSystem()
+    bad = []
+    for k, spec in enumerate(specs):
+        g = S.build_generated_group(ss, spec, "set11_%d" % k)
+        n = 300 + 37 * k
+        w = G.group_inputs(n, 500 + k)
+        t = {x: torch.from_numpy(w[x]).to(dev) for x in ("P", "N", "wo", "wi")}
+        out = torch.full((4, n), -7.0, device=dev)
+        ev = torch.full((4, n), -7.0, device=dev)
+        glob = {"P": t["P"], "N": t["N"], "dPdx": (0.0, 0.0, 0.0), "dPdy": (0.0, 0.0, 0.0)}
+        ss.execute(g, n, glob, out, t["wo"], t["wi"], ev)
+        torch.cuda.synchronize()
+        ref_out, ref_ev = O.Group(spec[0], spec[1]).run(w["P"], w["N"], w["wo"], w["wi"],
+                                                        nthreads=8)
+        ok_o = close(out.cpu().numpy(), ref_out)
+        ok_e = close(ev.cpu().numpy(), ref_ev)
+        if not (ok_o.all() and ok_e.all()):
+            bad.append((k, int((~ok_o).sum()), int((~ok_e).sum())))
+    assert not bad, "groups with mismatches (group, outputs, evals): %s" % bad
