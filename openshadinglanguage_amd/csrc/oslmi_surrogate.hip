@@ -258,20 +258,35 @@ __global__ __launch_bounds__(256) void k_reduce_grads(const float* __restrict__ 
     while (l < 5 && p >= OFF_W[l + 1])
         l++;
     const int nw = LAYER_OUT[l] * LAYER_IN[l];
+    if (p >= OFF_B[l])
+        return;  // biases: k_reduce_bias
     float g = 0.0f;
-    if (p < OFF_B[l]) {
-        const int e = p - OFF_W[l];
-        const float* slab = wpart + (int64_t)nsplit * OFF_W[l];
-        for (int k = 0; k < nsplit; k++)
-            g += slab[(int64_t)k * nw + e];
-    } else {
-        const int o = p - OFF_B[l];
-        const int nout = LAYER_OUT[l];
-        const float* slab = bpart + (int64_t)nblk * (l - 1) * H;
-        for (int k = 0; k < nblk; k++)
-            g += slab[(int64_t)k * nout + o];
-    }
+    const int e = p - OFF_W[l];
+    const float* slab = wpart + (int64_t)nsplit * OFF_W[l];
+    for (int k = 0; k < nsplit; k++)
+        g += slab[(int64_t)k * nw + e];
     G[p] = g;
+}
+
+// Bias gradients: one wave per bias, each lane summing every 64th
+// per-block partial, then a fixed-order shuffle reduction (deterministic).
+__global__ __launch_bounds__(64) void k_reduce_bias(const float* __restrict__ bpart, int nblk,
+                                                    float* __restrict__ G)
+{
+    int l = 1, o = blockIdx.x;
+    while (l < 5 && o >= LAYER_OUT[l]) {
+        o -= LAYER_OUT[l];
+        l++;
+    }
+    const int nout    = LAYER_OUT[l];
+    const float* slab = bpart + (int64_t)nblk * (l - 1) * H;
+    float g = 0.0f;
+    for (int k = threadIdx.x; k < nblk; k += 64)
+        g += slab[(int64_t)k * nout + o];
+    for (int off = 32; off > 0; off >>= 1)
+        g += __shfl_down(g, off);
+    if (threadIdx.x == 0)
+        G[OFF_B[l] + o] = g;
 }
 
 // Adam on FP32 master weights; refreshes the bf16 W and W^T copies.
@@ -820,6 +835,8 @@ int fwd_bwd(Context* c, int64_t B64, const float* q, const float* t, int64_t ldq
     }
     hipLaunchKernelGGL(k_reduce_grads, dim3((NPARAM + 255) / 256), dim3(256), 0, st, c->wpart,
                        c->bpart, nsplit, nblk, c->G);
+    hipLaunchKernelGGL(k_reduce_bias, dim3(4 * H + NOUT_PAD), dim3(64), 0, st, c->bpart, nblk,
+                       c->G);
     return check_launch("surrogate fwd/bwd");
 }
 
