@@ -427,7 +427,7 @@ def main():
                     help="skip the 64-group batch measurement")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 256x256 single-layer group (graph-replayed) measurement")
-    ap.add_argument("--train-steps", type=int, default=60000)
+    ap.add_argument("--train-steps", type=int, default=120000)
     # 6 passes over the full workload: about 1 s wall, 15-20 s of CPU work
     # on 16 host threads
     ap.add_argument("--cpu-passes", type=int, default=6)
