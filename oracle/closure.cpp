@@ -83,7 +83,7 @@ float ggx_D(V3 Hr, float xa, float ya)
 BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
 {
     if (l.kind == kLobeDiffuse) {
-        const float pdf = std::max(dot(l.N, wi), 0.0f) * float(M_1_PI);
+        const float pdf = std::max<float>(dot(l.N, wi), 0.0f) * float(M_1_PI);
         return { V3(1.0f), pdf };
     }
     Frame tf        = frame_from_normal_tangent(l.N, V3(0.0f));

@@ -300,7 +300,7 @@ D2<float> cell(const GaborUniform& gup, const GaborOpts& opt, const Params& gp, 
                 float omega_t = kTwoPi * rng();
                 float u       = rng();
                 float cos_p   = -1.0f * (1.0f - u) + 1.0f * u;  // OIIO::lerp(-1,1,u)
-                float sin_p   = sqrtf(std::max(0.0f, 1.0f - cos_p * cos_p));
+                float sin_p   = sqrtf(std::max<float>(0.0f, 1.0f - cos_p * cos_p));
                 float st, ct;
                 ofast_sincosf(omega_t, &st, &ct);
                 omega = normalized(V3(ct * sin_p, st * sin_p, cos_p));
@@ -355,7 +355,7 @@ D2<float> cell(const GaborUniform& gup, const GaborOpts& opt, const Params& gp, 
 GaborUniform gabor_uniform(const GaborOpts& opt)
 {
     GaborUniform g;
-    float bandwidth = std::min(std::max(opt.bandwidth, 0.01f), 100.0f);
+    float bandwidth = std::min<float>(std::max<float>(opt.bandwidth, 0.01f), 100.0f);
     float two_bw    = exp2f(bandwidth);
     const float SQRT_PI_OVER_LN2 = 2.128934e+00f;
     g.a          = 2.0f * ((two_bw - 1.0f) / (two_bw + 1.0f)) * SQRT_PI_OVER_LN2;
@@ -363,7 +363,7 @@ GaborUniform gabor_uniform(const GaborOpts& opt)
     g.radius2    = g.radius * g.radius;
     g.radius3    = g.radius2 * g.radius;
     g.radius_inv = 1.0f / g.radius;
-    float imp    = std::min(std::max(opt.impulses, 1.0f), 32.0f);
+    float imp    = std::min<float>(std::max<float>(opt.impulses, 1.0f), 32.0f);
     g.lambda     = imp / (float(1.33333 * M_PI) * g.radius3);
     g.sqrt_lambda_inv = 1.0f / sqrtf(g.lambda);
     g.poisson_g       = expf(-(g.lambda * g.radius3));

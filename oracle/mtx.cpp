@@ -844,13 +844,22 @@ bool lut_load(const std::string& path, float* out)
     if (!f)
         return false;
     char magic[8];
+#ifdef OREF_F64
+    // the tables are binary32 data; the FP64 build widens them on load
+    std::vector<oref_f32> t(kLutFloats);
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kLutMagic, 8) == 0
+              && fread(t.data(), 4, kLutFloats, f) == (size_t)kLutFloats;
+    for (int i = 0; ok && i < kLutFloats; i++)
+        out[i] = t[i];
+#else
     bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kLutMagic, 8) == 0
               && fread(out, 4, kLutFloats, f) == (size_t)kLutFloats;
+#endif
     fclose(f);
     return ok;
 }
 
-#ifndef OREF_COUNTING
+#if !defined(OREF_COUNTING) && !defined(OREF_F64)
 uint32_t ri_LP(uint32_t i)
 {
     uint32_t r = 0;
@@ -964,7 +973,7 @@ template<class F> void parallel_for_n(int n, int nthreads, F fn)
 
 }  // namespace
 
-#ifndef OREF_COUNTING
+#if !defined(OREF_COUNTING) && !defined(OREF_F64)
 void bake_luts(float* out, int nthreads)
 {
     if (nthreads < 1)
@@ -1020,7 +1029,7 @@ const float* luts()
         g_lut.assign(kLutFloats, 0.0f);
         const std::string path = lut_cache_path();
         if (!lut_load(path, g_lut.data())) {
-#ifndef OREF_COUNTING
+#if !defined(OREF_COUNTING) && !defined(OREF_F64)
             unsigned hc = std::thread::hardware_concurrency();
             bake_luts(g_lut.data(), hc ? (int)(hc < 16 ? hc : 16) : 4);
             save_luts(path.c_str(), g_lut.data());

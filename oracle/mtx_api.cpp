@@ -151,7 +151,7 @@ void oref_mtx_luts(float* out)
         out[i] = t[i];
 }
 
-#ifndef OREF_COUNTING
+#if !defined(OREF_COUNTING) && !defined(OREF_F64)
 void oref_mtx_bake(float* out, int nthreads) { M::bake_luts(out, nthreads); }
 #endif
 

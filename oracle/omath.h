@@ -15,6 +15,40 @@
 
 namespace oref {
 
+#ifdef OREF_F64
+// FP64 evaluation build (f64/f64_main.cpp): `float` is binary64 there and
+// each approximation below is replaced by the exact function it stands
+// for, so that build evaluates the shading path's mathematics, not its
+// binary32 rounding or its polynomial truncation.
+inline float ofast_expf(float x) { return std::exp(x); }
+inline void ofast_sincosf(float x, float* s, float* c)
+{
+    *s = std::sin(x);
+    *c = std::cos(x);
+}
+inline float ofast_expm1f(float x) { return std::expm1(x); }
+inline float ofast_log2f(float x) { return std::log2(x); }
+inline float ofast_exp2f(float t) { return std::exp2(t); }
+inline float ofast_safe_powf(float x, float y)
+{
+    if (y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f)
+        return 0.0f;
+    if (y == 1.0f)
+        return x;
+    if (y == 2.0f) {
+        const float x2   = x * x;
+        const float fmax = 3.40282346638528859812e+38f;
+        return x2 < fmax ? x2 : fmax;
+    }
+    if (x < 0.0f)
+        return 0.0f;
+    return std::pow(x, y);
+}
+inline float ofast_cbrtf(float x) { return std::cbrt(x); }
+#else
+
 inline float bits_f(uint32_t u)
 {
     float f;
@@ -208,5 +242,6 @@ inline float ofast_cbrtf(float x)
     }
     return x < 0.0f ? -y : y;
 }
+#endif  // OREF_F64
 
 }  // namespace oref

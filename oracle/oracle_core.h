@@ -106,6 +106,24 @@ inline D2<float> operator/(const D2<float>& a, const D2<float>& b)
     return D2<float>(ab, binv * (a.dx - ab * b.dx), binv * (a.dy - ab * b.dy));
 }
 
+#ifdef OREF_F64
+// FP64 evaluation build: hashes still see the binary32 value of a
+// coordinate (oref_f32 is the real 4-byte float there), so lattice and
+// cell choices match the binary32 path wherever the coordinate rounds alike.
+inline uint32_t f2u(float f)
+{
+    const oref_f32 g = (oref_f32)f;
+    uint32_t u;
+    std::memcpy(&u, &g, 4);
+    return u;
+}
+inline float u2f(uint32_t u)
+{
+    oref_f32 g;
+    std::memcpy(&g, &u, 4);
+    return g;
+}
+#else
 inline uint32_t f2u(float f)
 {
     uint32_t u;
@@ -118,5 +136,6 @@ inline float u2f(uint32_t u)
     std::memcpy(&f, &u, 4);
     return f;
 }
+#endif
 
 }  // namespace oref

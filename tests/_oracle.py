@@ -271,3 +271,118 @@ def composite_eval(lobes, wo, wi):
     out = np.zeros((4, n), np.float32)
     L.oref_composite_eval(lo.shape[0], _p(lo), _p(a[0]), _p(a[1]), _p(out), n)
     return out
+
+
+# ---- FP64 evaluation build (oracle/f64/f64_main.cpp) -----------------------
+LIB64 = os.path.join(ROOT, "oracle", "build", "liboslref_f64.so")
+_lib64 = None
+
+
+def lib64():
+    """The oracle compiled in binary64 with exact transcendentals: the
+    mathematics of the shading path, against which binary32 results get an
+    error. Inputs are the binary32 inputs widened exactly."""
+    global _lib64
+    if _lib64 is None:
+        if not os.path.exists(LIB64):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "f64"])
+        L = ctypes.CDLL(LIB64)
+        P, I64, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        L.oref64_shade_c1.argtypes = [P, P, I64, I]
+        L.oref64_shade_c2.argtypes = [P] * 7 + [I64, I]
+        L.oref64_mtx_layered.argtypes = [P] * 8 + [I64, I]
+        L.oref64_math.argtypes = [I, P, P, I64]
+        _lib64 = L
+    return _lib64
+
+
+def _d(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32), np.float64)
+
+
+def shade_c1_f64(P, nthreads=1):
+    P = _d(P)
+    out = np.zeros_like(P)
+    lib64().oref64_shade_c1(_p(P), _p(out), P.shape[1], nthreads)
+    return out
+
+
+def shade_c2_f64(P, N, wo, wi, dPdx, dPdy, nthreads=1):
+    a = [_d(x) for x in (P, N, wo, wi, dPdx, dPdy)]
+    n = a[0].shape[1]
+    out = np.zeros((4, n), np.float64)
+    lib64().oref64_shade_c2(*[_p(x) for x in a], _p(out), n, nthreads)
+    return out
+
+
+def mtx_layered_f64(mat, N, V, L, params, rnd, nthreads=1):
+    """Returns (eval [4, n], samples [28, n]) in binary64."""
+    a = [_d(x) for x in (mat, N, V, L, params, rnd)]
+    n = a[1].shape[1]
+    ev = np.zeros((4, n), np.float64)
+    sp = np.zeros((28, n), np.float64)
+    lib64().oref64_mtx_layered(*[_p(x) for x in a], _p(ev), _p(sp), n, nthreads)
+    return ev, sp
+
+
+def omath_f64(fn, x):
+    """The FP64 build's exp/sin/cos/cbrt (the exact binary64 functions)."""
+    x = _d(x)
+    out = np.zeros_like(x)
+    lib64().oref64_math(["exp", "sin", "cos", "cbrt"].index(fn), _p(x), _p(out), x.size)
+    return out
+
+
+def f32_error(got, ref64):
+    """Error of binary32 results against a binary64 evaluation, per element:
+    (relative error, error in units of the binary32 spacing at the exact
+    value). Non-finite pairs count as exact only when both sides agree."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref64, np.float64)
+    d = np.abs(got - ref)
+    both = ~np.isfinite(got) | ~np.isfinite(ref)
+    d[both] = np.where(got[both] == ref[both], 0.0, np.inf)
+    mag = np.abs(ref)
+    rel = d / np.maximum(mag, 1e-30)
+    e = np.floor(np.log2(np.maximum(mag, np.float64(np.finfo(np.float32).tiny))))
+    ulp = d / np.exp2(e - 23)
+    return rel, ulp
+
+
+def error_summary(got, ref64):
+    """Percentiles of the binary32-vs-binary64 error over all elements."""
+    rel, ulp = f32_error(got, ref64)
+    rel, ulp = rel.ravel(), ulp.ravel()
+    q = [50, 99, 99.9]
+    s = {"n": int(rel.size)}
+    for name, v in (("rel", rel), ("ulp", ulp)):
+        fin = v[np.isfinite(v)]
+        for p, x in zip(q, np.percentile(fin, q)):
+            s["%s_p%s" % (name, str(p).replace(".", "_"))] = float(x)
+        s[name + "_max"] = float(fin.max())
+        s[name + "_mean"] = float(fin.mean())
+    s["frac_ulp_le4"] = float(np.mean(ulp <= 4.0))
+    s["frac_rel_le1e5"] = float(np.mean((ulp <= 4.0) | (rel <= 1e-5)))
+    return s
+
+
+def sample_flips(samp, samp64, tol=1e-3):
+    """Per (sample, point) of a [28, n] sample block (4 samples x wi xyz,
+    weight rgb, pdf): True where binary32 rounding changed a discrete
+    choice (lobe, reflection or refraction, rejection), seen as a component
+    that moved by more than tol relative to max(|exact|, 1)."""
+    a = np.asarray(samp, np.float64).reshape(4, 7, -1)
+    b = np.asarray(samp64, np.float64).reshape(4, 7, -1)
+    d = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+    d[~np.isfinite(d)] = np.inf
+    return np.any(d > tol, axis=1)
+
+
+def sample_error_summary(samp, samp64):
+    """error_summary over the samples whose discrete choices agree, plus
+    the fraction of samples whose choices differ."""
+    flip = sample_flips(samp, samp64)
+    keep = np.broadcast_to(~flip[:, None, :], (4, 7, flip.shape[1])).reshape(28, -1)
+    s = error_summary(np.asarray(samp)[keep], np.asarray(samp64)[keep])
+    s["frac_flipped"] = float(flip.mean())
+    return s
