@@ -78,7 +78,10 @@ typedef struct oslmi_batched_globals {
  * the JIT; cache_dir: code-object cache directory or NULL. */
 oslmi_shadingsys* oslmi_shadingsys_create(const char* include_dir, const char* cache_dir);
 void oslmi_shadingsys_destroy(oslmi_shadingsys* ss);
-/* ShadingSystem::attribute (oslexec.h:177-324): "colorspace", "jit_cache". */
+/* ShadingSystem::attribute (oslexec.h:177-324): "colorspace", "jit_cache",
+ * "include_dir", and "gpu_waves_per_eu" (0..8: the occupancy, in waves per
+ * SIMD, the emitted kernels are register-allocated for; default 6, 0 leaves
+ * it to the compiler). */
 int oslmi_shadingsys_attribute(oslmi_shadingsys* ss, const char* name, const char* value);
 /* LoadMemoryCompiledShader: register a shader master from OSO text. */
 int oslmi_load_memory_compiled_shader(oslmi_shadingsys* ss, const char* shadername,

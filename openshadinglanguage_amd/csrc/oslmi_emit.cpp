@@ -49,6 +49,17 @@ std::string flit(float f)
     return buf;
 }
 
+// launch bounds of every emitted kernel (256 work-items per workgroup) and,
+// when the shading system sets one, the occupancy target the register
+// allocator works to
+std::string kernel_attributes(const ShadingSys& ss)
+{
+    std::string a = "__launch_bounds__(256)";
+    if (ss.waves_per_eu > 0)
+        a += " __attribute__((amdgpu_waves_per_eu(" + std::to_string(ss.waves_per_eu) + ")))";
+    return a;
+}
+
 enum Kind { K_INT, K_FLOAT, K_DF, K_V3, K_DV, K_MAT, K_CLOSURE, K_STRING, K_VOID };
 
 struct Emitter {
@@ -1122,7 +1133,7 @@ struct Emitter {
         std::ostringstream k;
         k << "// Emitted by the oslmi group emitter for group '" << g.name << "'.\n";
         k << prelude();
-        k << "extern \"C\" __global__ __launch_bounds__(256) void oslmi_group_main(GroupArgs A)\n{\n";
+        k << "extern \"C\" __global__ " << kernel_attributes(ss) << " void oslmi_group_main(GroupArgs A)\n{\n";
         k << "    using Closure = LobeList<" << closure_capacity() << ">;\n";
         if (uses_cs)
             k << "    const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
@@ -1222,7 +1233,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
          "    if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1);\n"
          "    __syncthreads();\n"
          "    return *s_tile;\n}\n";
-    k << "extern \"C\" __global__ __launch_bounds__(256) void oslmi_batch_main(BatchArgs B)\n{\n";
+    k << "extern \"C\" __global__ " << kernel_attributes(ss) << " void oslmi_batch_main(BatchArgs B)\n{\n";
     k << "    const GroupArgs* __restrict__ args = B.args;\n";
     k << "    __shared__ int s_tile;\n";
     if (any_bb) {

@@ -73,6 +73,7 @@ struct ShadingSys {
     std::string colorspace = "Rec709";
     std::string cache_dir;   // code-object cache ("" = none)
     std::string include_dir; // device library headers for hiprtc
+    int waves_per_eu = 6;    // occupancy target of emitted kernels (0: compiler default)
     std::mutex mu;
 };
 

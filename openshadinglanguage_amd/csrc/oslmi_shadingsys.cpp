@@ -254,6 +254,14 @@ int oslmi_shadingsys_attribute(oslmi_shadingsys* s, const char* name, const char
         s->ss.cache_dir = value;
     } else if (n == "include_dir") {
         s->ss.include_dir = value;
+    } else if (n == "gpu_waves_per_eu") {
+        // occupancy target (waves per SIMD) the emitted kernels are
+        // register-allocated for; 0 leaves it to the compiler
+        char* end = nullptr;
+        long w    = strtol(value, &end, 10);
+        if (!end || end == value || *end || w < 0 || w > 8)
+            return set_error(OSLMI_ERR_INVALID, "gpu_waves_per_eu must be 0..8");
+        s->ss.waves_per_eu = (int)w;
     } else {
         return set_error(OSLMI_ERR_UNSUPPORTED, "unknown attribute " + n);
     }

@@ -30,6 +30,18 @@ def test_emit_layered3_derivative_analysis():
     assert "blackbody_rgb(s_bb" in src and "composite_eval" in src
 
 
+def test_occupancy_attribute():
+    ss, g = S.build_fbm_cell()
+    assert "amdgpu_waves_per_eu(6)" in ss.emit(g)  # the default target
+    ss.attribute("gpu_waves_per_eu", 0)
+    assert "amdgpu_waves_per_eu" not in ss.emit(g)
+    ss.attribute("gpu_waves_per_eu", 4)
+    assert "amdgpu_waves_per_eu(4)" in ss.emit(g)
+    for bad in ("9", "-1", "six", ""):
+        with pytest.raises(OSLMIError):
+            ss.attribute("gpu_waves_per_eu", bad)
+
+
 def test_hiprtc_compile_both_groups(tmp_path):
     for build in (S.build_fbm_cell, S.build_layered3):
         ss, g = build(S.ShadingSystem(jit_cache=str(tmp_path)))
