@@ -42,8 +42,9 @@ WORKLOADS = {
 # HBM traffic per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (profiles/*_pmc.txt), in bytes; None when not measured.
 # FETCH_SIZE reads half the bytes of a plain 256 MiB copy on this GPU (x2
-# scale), WRITE_SIZE reads them exactly (x1); profiles/layered3_pmc.txt.
-MEASURED_TRAFFIC = {"layered3": 2 * 24797.7 * 1024 + 16384.0 * 1024, "fbm_cell": None}
+# scale), WRITE_SIZE reads them exactly (x1); profiles/layered3_pmc.txt. The
+# layered3 figure includes the kernel's register-spill (scratch) traffic.
+MEASURED_TRAFFIC = {"layered3": 2 * 41121.9 * 1024 + 85283.5 * 1024, "fbm_cell": None}
 
 
 def _ops_per_point(name):
