@@ -83,6 +83,20 @@ void oslmi_shadingsys_destroy(oslmi_shadingsys* ss);
  * SIMD, the emitted kernels are register-allocated for; default 6, 0 leaves
  * it to the compiler). */
 int oslmi_shadingsys_attribute(oslmi_shadingsys* ss, const char* name, const char* value);
+/* Named coordinate systems, the renderer services behind transform(),
+ * matrix(space) and getmatrix(): RendererServices::get_matrix /
+ * get_inverse_matrix for a space name (batched_rendererservices.h:62-108,
+ * testshade's SimpleRenderer::name_transform, src/testshade/simplerend.cpp:
+ * 464-468), and the uniform shader2common / object2common matrices of a
+ * batch under the names "shader" and "object" (testshade.cpp:973-979).
+ * m: the space-to-common matrix, 16 floats in Imath's row-vector layout
+ * (translation in m[12..14]); inv: its inverse, or NULL to have the library
+ * invert m as OSL's default get_inverse_matrix does. "common" and the
+ * commonspace synonym ("world") are the identity and cannot be named.
+ * Takes effect at the next oslmi_execute_batch / oslmi_batch_bind, which
+ * then waits for the device once to update the table. */
+int oslmi_renderer_name_transform(oslmi_shadingsys* ss, const char* name, const float* m,
+                                  const float* inv);
 /* LoadMemoryCompiledShader: register a shader master from OSO text. */
 int oslmi_load_memory_compiled_shader(oslmi_shadingsys* ss, const char* shadername,
                                       const char* oso_text);

@@ -561,8 +561,8 @@ int oslmi_batch_bind(oslmi_batch* h, int nitems, const oslmi_batch_item* items)
     for (int k = 0; k < nitems; ++k) {
         const oslmi_batch_item& it = items[k];
         std::string who = "oslmi_batch_bind item " + std::to_string(k);
-        if (int rc = make_group_args(*b.groups[k], &it.globals, it.d_outputs, it.d_wo, it.d_wi,
-                                     it.d_eval, who.c_str(), args[k]))
+        if (int rc = make_group_args(*b.ss, *b.groups[k], &it.globals, it.d_outputs, it.d_wo,
+                                     it.d_wi, it.d_eval, who.c_str(), args[k]))
             return rc;
         if (it.globals.n > ((int64_t)1 << 40))
             return set_error(OSLMI_ERR_INVALID, who + ": n too large");

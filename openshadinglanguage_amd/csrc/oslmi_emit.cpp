@@ -77,6 +77,19 @@ struct Emitter {
     Emitter(const ShadingSys& s, Group& gr) : ss(s), g(gr) {}
 
     std::string lab(int n) const { return lprefix + std::to_string(n); }
+
+    // the transform-table slot of a (from, to) space pair, as an expression
+    std::string xform_ref(const std::string& from, const std::string& to)
+    {
+        const int k = xform_slot(ss, from, to);
+        if (k < 0) {
+            fail("too many distinct coordinate-system pairs (at most " +
+                 std::to_string(kMaxXformSlots) + ")");
+            return "";
+        }
+        g.uses_xforms = true;
+        return "A.xf + " + std::to_string(kXformSlotFloats * k);
+    }
     const ShaderMaster& M(int L) const { return *g.layers[L].master; }
     const OSymbol& S(int L, int s) const { return M(L).syms[s]; }
 
@@ -225,7 +238,8 @@ struct Emitter {
         static const std::set<std::string> d = {
             "assign", "add", "sub", "mul", "div", "neg", "mix", "smoothstep", "clamp", "min", "max",
             "sqrt", "exp", "sin", "cos", "dot", "compref", "compassign", "color", "point", "vector",
-            "normal", "transformc", "noise", "snoise", "pnoise", "psnoise"
+            "normal", "transformc", "noise", "snoise", "pnoise", "psnoise", "transform",
+            "transformv", "transformn"
         };
         return d.count(o) > 0;
     }
@@ -852,15 +866,43 @@ struct Emitter {
             line(R + " = " + v + ";");
             return;
         }
-        if (o == "transform") {
-            // transform r M p (matrix form)
-            if (op.args.size() == 3 && S(L, a(1)).type.base == kMatrix) {
-                std::string fn = S(L, a(2)).type.triple_kind == "point" ? "xform_point" : "xform_vector";
-                std::string v = fn + "(" + var(L, a(1)) + ", " + ex(L, a(2), K_V3) + ")";
-                line(R + " = " + (rk == K_DV ? "Dv(" + v + ")" : v) + ";");
+        if (o == "transform" || o == "transformv" || o == "transformn") {
+            // transform[v|n] R [from] to P, or R M P (batched_llvm_gen.cpp:3470-3560):
+            // the opcode names the semantics (point, vector, normal)
+            const std::string kind = o == "transform" ? "point" : (o == "transformv" ? "vector" : "normal");
+            const int nargs = (int)op.args.size();
+            if (nargs != 3 && nargs != 4) {
+                fail(o + " needs 3 or 4 arguments");
                 return;
             }
-            fail("transform by named spaces is not supported");
+            const int pi = nargs == 3 ? 2 : 3;
+            const std::string p = ex(L, a(pi), rk == K_DV ? K_DV : K_V3);
+            if (nargs == 3 && S(L, a(1)).type.base == kMatrix) {
+                const std::string fn = "xform_" + kind;
+                line(R + " = " + fn + "(" + var(L, a(1)) + ", " + p + ");");
+                return;
+            }
+            std::string from = "common", to;
+            if ((nargs == 4 && !strval(L, a(1), from)) || !strval(L, a(nargs == 3 ? 1 : 2), to)) {
+                fail(o + " needs constant space names");
+                return;
+            }
+            if (from == ss.commonspace)
+                from = "common";
+            if (to == ss.commonspace)
+                to = "common";
+            if (from == to) {  // identity: a copy
+                line(R + " = " + p + ";");
+                return;
+            }
+            const std::string slot = xform_ref(from, to);
+            if (slot.empty())
+                return;
+            // unknown spaces leave the value unchanged (opmatrix.cpp:286-292)
+            const std::string fn = kind == "normal" ? "xform_vector" : "xform_" + kind;
+            const std::string mat = kind == "normal" ? "xf_normal_matrix(" + slot + ")"
+                                                     : "xf_matrix(" + slot + ")";
+            line(R + " = xf_ok(" + slot + ") ? " + fn + "(" + mat + ", " + p + ") : " + p + ";");
             return;
         }
         if (o == "matrix") {
@@ -871,7 +913,47 @@ struct Emitter {
                 line(m + "}};");
                 return;
             }
+            // matrix(from) and matrix(from, to) (opmatrix.cpp:141-243)
+            std::string from, to = "common";
+            if ((op.args.size() == 2 || op.args.size() == 3) && strval(L, a(1), from)
+                && (op.args.size() == 2 || strval(L, a(2), to))) {
+                if (from == ss.commonspace)
+                    from = "common";
+                if (to == ss.commonspace)
+                    to = "common";
+                if (from == to) {
+                    line(R + " = m44_identity();");
+                    return;
+                }
+                const std::string slot = xform_ref(from, to);
+                if (!slot.empty())
+                    line(R + " = xf_matrix(" + slot + ");");
+                return;
+            }
             fail("unsupported matrix constructor");
+            return;
+        }
+        if (o == "getmatrix") {
+            // int getmatrix(from, to, output M) (osl_get_from_to_matrix)
+            std::string from, to;
+            if (op.args.size() != 4 || !strval(L, a(1), from) || !strval(L, a(2), to)) {
+                fail("getmatrix needs constant space names");
+                return;
+            }
+            if (from == ss.commonspace)
+                from = "common";
+            if (to == ss.commonspace)
+                to = "common";
+            if (from == to) {
+                line(var(L, a(3)) + " = m44_identity();");
+                line(R + " = 1;");
+                return;
+            }
+            const std::string slot = xform_ref(from, to);
+            if (slot.empty())
+                return;
+            line(var(L, a(3)) + " = xf_matrix(" + slot + ");");
+            line(R + " = xf_ok(" + slot + ") ? 1 : 0;");
             return;
         }
         if (o == "closure") {
@@ -993,6 +1075,7 @@ struct Emitter {
                 return false;
             }
         analyze();
+        g.uses_xforms = false;
         // outputs
         g.outputs.clear();
         int plane = 0;

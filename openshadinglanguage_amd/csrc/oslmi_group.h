@@ -17,6 +17,7 @@
 
 #include "oslmi.h"
 #include "oslmi_groupargs.h"
+#include "oslmi_matrix.h"
 #include "oslmi_oso.h"
 
 namespace oslmi {
@@ -59,6 +60,7 @@ struct Group {
     int output_planes        = 0;
     bool surface_closure     = false;  // last layer writes Ci
     bool uses_colorsys       = false;
+    bool uses_xforms         = false;  // reads the transform table
     std::vector<unsigned> globals_used;  // GlobalIndex values read
     hipModule_t module       = nullptr;
     hipFunction_t kernel     = nullptr;
@@ -67,15 +69,49 @@ struct Group {
     int find_layer(const std::string& n) const;
 };
 
+// A named coordinate system the renderer registers: the matrix to "common"
+// space (RendererServices::get_matrix) and, optionally, its inverse
+// (get_inverse_matrix); without one the inverse is computed.
+struct NamedXform {
+    float m[16];
+    float inv[16];
+    bool has_inv = false;
+};
+
+// Transform table: one slot per (from, to) space pair some emitted kernel
+// uses, kXformSlotFloats floats each: the from-to matrix (16), the
+// transposed inverse for normals (16) and 1/0 for whether both spaces
+// were known (1).
+constexpr int kXformSlotFloats = 33;
+constexpr int kMaxXformSlots   = 256;
+
+struct XformTable {
+    std::mutex mu;
+    std::map<std::string, NamedXform> named;
+    std::vector<std::pair<std::string, std::string>> pairs;
+    float* device = nullptr;  // kMaxXformSlots slots, allocated on first use
+    bool dirty    = true;     // device copy out of date
+};
+
 struct ShadingSys {
     std::map<std::string, std::unique_ptr<ShaderMaster>> masters;
     std::vector<std::unique_ptr<Group>> groups;
     std::string colorspace = "Rec709";
+    std::string commonspace = "world";  // synonym of "common"
     std::string cache_dir;   // code-object cache ("" = none)
     std::string include_dir; // device library headers for hiprtc
     int waves_per_eu = 6;    // occupancy target of emitted kernels (0: compiler default)
+    mutable XformTable xf;
     std::mutex mu;
 };
+
+// Slot of the (from, to) pair in the shading system's transform table,
+// added on first use; -1 when the table is full.
+int xform_slot(const ShadingSys& ss, const std::string& from, const std::string& to);
+
+// Brings the device copy of the transform table up to date (allocating it
+// on first use); a no-op when nothing changed.
+int xform_table_sync(const ShadingSys& ss);
 
 // Emits the HIP source of the group's fused kernel; fills g.outputs,
 // g.output_planes, g.surface_closure, g.uses_colorsys, g.globals_used.
@@ -97,9 +133,9 @@ int device_colorsys(const ColorSysDev** out);
 
 // Kernel argument block of one execution of group g (validates the
 // arguments like BatchedExecutor::execute; `who` prefixes error messages).
-int make_group_args(const Group& g, const oslmi_batched_globals* bg, float* d_outputs,
-                    const float* d_wo, const float* d_wi, float* d_eval, const char* who,
-                    GroupArgs& A);
+int make_group_args(const ShadingSys& ss, const Group& g, const oslmi_batched_globals* bg,
+                    float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
+                    const char* who, GroupArgs& A);
 }  // namespace oslmi
 
 // Opaque handles of the C ABI.

@@ -28,6 +28,8 @@ struct GroupArgs {
     const float* wo;
     const float* wi;
     float* eval;
+    // the shading system's transform table (33 floats per space pair)
+    const float* xf;
 };
 
 // One unit of work of the persistent batch kernel: points [begin, end) of

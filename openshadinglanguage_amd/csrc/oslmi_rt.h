@@ -14,6 +14,7 @@
 #include "oslmi_color.h"
 #include "oslmi_gabor.h"
 #include "oslmi_math.h"
+#include "oslmi_matrix.h"
 #include "oslmi_noise.h"
 
 namespace oslmi {
@@ -185,26 +186,7 @@ OSLMI_DEV V3 triple(float a, float b, float c) { return V3(a, b, c); }
 OSLMI_DEV Dv triple(const Df& a, const Df& b, const Df& c) { return make_dv(a, b, c); }
 
 // ---------------------------------------------------------- matrices
-struct M44 {
-    float m[16];
-};
-// transform of a point (w = 1, homogeneous divide), vector, normal by a
-// row-vector convention matrix (Imath::Matrix44 multVecMatrix/multDirMatrix).
-OSLMI_DEV V3 xform_point(const M44& M, V3 p)
-{
-    const float* m = M.m;
-    float a = p.x * m[0] + p.y * m[4] + p.z * m[8] + m[12];
-    float b = p.x * m[1] + p.y * m[5] + p.z * m[9] + m[13];
-    float c = p.x * m[2] + p.y * m[6] + p.z * m[10] + m[14];
-    float w = p.x * m[3] + p.y * m[7] + p.z * m[11] + m[15];
-    return V3(a / w, b / w, c / w);
-}
-OSLMI_DEV V3 xform_vector(const M44& M, V3 p)
-{
-    const float* m = M.m;
-    return V3(p.x * m[0] + p.y * m[4] + p.z * m[8], p.x * m[1] + p.y * m[5] + p.z * m[9],
-              p.x * m[2] + p.y * m[6] + p.z * m[10]);
-}
+// M44 and the point / vector / normal transforms: oslmi_matrix.h
 
 // ---------------------------------------------------------- closures
 // A closure value is a LobeList whose capacity the emitter bounds per

@@ -99,7 +99,8 @@ int build_code_object(const ShadingSys& ss, const std::string& src, const std::s
     // include the device library headers in the cache key
     std::string hdrs;
     for (const char* h : { "oslmi_rt.h", "oslmi_types.h", "oslmi_noise.h", "oslmi_gabor.h",
-                           "oslmi_math.h", "oslmi_color.h", "oslmi_closure.h", "oslmi_groupargs.h" }) {
+                           "oslmi_math.h", "oslmi_color.h", "oslmi_closure.h", "oslmi_groupargs.h",
+                           "oslmi_matrix.h" }) {
         std::vector<char> c;
         read_file(ss.include_dir + "/" + h, c);
         hdrs.append(c.begin(), c.end());
@@ -147,9 +148,9 @@ int build_code_object(const ShadingSys& ss, const std::string& src, const std::s
     return OSLMI_OK;
 }
 
-int make_group_args(const Group& g, const oslmi_batched_globals* bg, float* d_outputs,
-                    const float* d_wo, const float* d_wi, float* d_eval, const char* who,
-                    GroupArgs& A)
+int make_group_args(const ShadingSys& ss, const Group& g, const oslmi_batched_globals* bg,
+                    float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
+                    const char* who, GroupArgs& A)
 {
     std::string w(who);
     if (bg->n < 0)
@@ -177,6 +178,114 @@ int make_group_args(const Group& g, const oslmi_batched_globals* bg, float* d_ou
             return rc;
         A.colorsys = cs;
     }
+    if (g.uses_xforms) {
+        if (int rc = xform_table_sync(ss))
+            return rc;
+        A.xf = ss.xf.device;
+    }
+    return OSLMI_OK;
+}
+
+// ---- renderer services: named coordinate systems ---------------------------
+namespace {
+
+bool is_common(const ShadingSys& ss, const std::string& n)
+{
+    return n == "common" || n == ss.commonspace;
+}
+// RendererServices::get_matrix(from): to-common matrix; identity and false
+// for an unknown space (wide_opmatrix.cpp:533-568)
+bool get_matrix(const ShadingSys& ss, const std::string& from, M44& M)
+{
+    M = m44_identity();
+    if (is_common(ss, from))
+        return true;
+    auto it = ss.xf.named.find(from);
+    if (it == ss.xf.named.end())
+        return false;
+    memcpy(M.m, it->second.m, sizeof(M.m));
+    return true;
+}
+// get_inverse_matrix(to): the renderer's inverse, or the matrix inverted as
+// the default implementation does (wide_opmatrix.cpp:134-150, 573-610)
+bool get_inverse_matrix(const ShadingSys& ss, const std::string& to, M44& M)
+{
+    M = m44_identity();
+    if (is_common(ss, to))
+        return true;
+    auto it = ss.xf.named.find(to);
+    if (it == ss.xf.named.end())
+        return false;
+    if (it->second.has_inv) {
+        memcpy(M.m, it->second.inv, sizeof(M.m));
+    } else {
+        M44 f;
+        memcpy(f.m, it->second.m, sizeof(f.m));
+        M = m44_inverse(f);
+    }
+    return true;
+}
+// the from-to matrix of osl_transform_triple / osl_get_from_to_matrix
+// (src/liboslexec/opmatrix.cpp:232-262)
+bool from_to_matrix(const ShadingSys& ss, const std::string& from, const std::string& to, M44& M)
+{
+    M44 mf, mt;
+    if (is_common(ss, from))
+        return get_inverse_matrix(ss, to, M);
+    if (is_common(ss, to))
+        return get_matrix(ss, from, M);
+    bool ok = get_matrix(ss, from, mf);
+    ok      = get_inverse_matrix(ss, to, mt) && ok;
+    M       = m44_mul(mf, mt);
+    return ok;
+}
+
+}  // namespace
+
+int xform_slot(const ShadingSys& ss, const std::string& from, const std::string& to)
+{
+    std::lock_guard<std::mutex> lock(ss.xf.mu);
+    for (size_t k = 0; k < ss.xf.pairs.size(); ++k)
+        if (ss.xf.pairs[k].first == from && ss.xf.pairs[k].second == to)
+            return (int)k;
+    if ((int)ss.xf.pairs.size() >= kMaxXformSlots)
+        return -1;
+    ss.xf.pairs.emplace_back(from, to);
+    ss.xf.dirty = true;
+    return (int)ss.xf.pairs.size() - 1;
+}
+
+int xform_table_sync(const ShadingSys& ss)
+{
+    std::lock_guard<std::mutex> lock(ss.xf.mu);
+    if (!ss.xf.dirty && ss.xf.device)
+        return OSLMI_OK;
+    if (!ss.xf.device) {
+        if (int rc = check_hip(hipMalloc((void**)&ss.xf.device,
+                                         sizeof(float) * kXformSlotFloats * kMaxXformSlots),
+                               "hipMalloc(transform table)"))
+            return rc;
+    }
+    std::vector<float> host(kXformSlotFloats * ss.xf.pairs.size());
+    for (size_t k = 0; k < ss.xf.pairs.size(); ++k) {
+        M44 M;
+        const bool ok = from_to_matrix(ss, ss.xf.pairs[k].first, ss.xf.pairs[k].second, M);
+        const M44 N   = m44_transposed(m44_inverse(M));
+        float* slot   = host.data() + kXformSlotFloats * k;
+        memcpy(slot, M.m, sizeof(M.m));
+        memcpy(slot + 16, N.m, sizeof(N.m));
+        slot[32] = ok ? 1.0f : 0.0f;
+    }
+    // kernels of earlier launches may still read the table
+    if (int rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))
+        return rc;
+    if (!host.empty()) {
+        if (int rc = check_hip(hipMemcpy(ss.xf.device, host.data(), sizeof(float) * host.size(),
+                                         hipMemcpyHostToDevice),
+                               "hipMemcpy(transform table)"))
+            return rc;
+    }
+    ss.xf.dirty = false;
     return OSLMI_OK;
 }
 
@@ -238,6 +347,8 @@ void oslmi_shadingsys_destroy(oslmi_shadingsys* s)
     for (auto& g : s->ss.groups)
         if (g->module)
             (void)hipModuleUnload(g->module);
+    if (s->ss.xf.device)
+        (void)hipFree(s->ss.xf.device);
     delete s;
 }
 
@@ -254,6 +365,14 @@ int oslmi_shadingsys_attribute(oslmi_shadingsys* s, const char* name, const char
         s->ss.cache_dir = value;
     } else if (n == "include_dir") {
         s->ss.include_dir = value;
+    } else if (n == "commonspace") {
+        // the name that is a synonym of "common" (ShadingSystem attribute
+        // "commonspace", default "world")
+        if (!*value)
+            return set_error(OSLMI_ERR_INVALID, "commonspace must be a name");
+        std::lock_guard<std::mutex> lock(s->ss.xf.mu);
+        s->ss.commonspace = value;
+        s->ss.xf.dirty    = true;
     } else if (n == "gpu_waves_per_eu") {
         // occupancy target (waves per SIMD) the emitted kernels are
         // register-allocated for; 0 leaves it to the compiler
@@ -265,6 +384,25 @@ int oslmi_shadingsys_attribute(oslmi_shadingsys* s, const char* name, const char
     } else {
         return set_error(OSLMI_ERR_UNSUPPORTED, "unknown attribute " + n);
     }
+    return OSLMI_OK;
+}
+
+int oslmi_renderer_name_transform(oslmi_shadingsys* s, const char* name, const float* m,
+                                  const float* inv)
+{
+    if (!s || !name || !m)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_renderer_name_transform: null argument");
+    std::string n(name);
+    if (n.empty() || n == "common" || n == s->ss.commonspace)
+        return set_error(OSLMI_ERR_INVALID,
+                         "oslmi_renderer_name_transform: \"" + n + "\" is the common space");
+    std::lock_guard<std::mutex> lock(s->ss.xf.mu);
+    NamedXform& x = s->ss.xf.named[n];
+    memcpy(x.m, m, sizeof(x.m));
+    x.has_inv = inv != nullptr;
+    if (inv)
+        memcpy(x.inv, inv, sizeof(x.inv));
+    s->ss.xf.dirty = true;
     return OSLMI_OK;
 }
 
@@ -447,7 +585,8 @@ int oslmi_execute_batch(oslmi_shadingsys* s, oslmi_group* gh, const oslmi_batche
             return rc;
     }
     GroupArgs A;
-    if (int rc = make_group_args(g, bg, d_outputs, d_wo, d_wi, d_eval, "oslmi_execute_batch", A))
+    if (int rc = make_group_args(s->ss, g, bg, d_outputs, d_wo, d_wi, d_eval,
+                                 "oslmi_execute_batch", A))
         return rc;
     if (bg->n == 0)
         return OSLMI_OK;

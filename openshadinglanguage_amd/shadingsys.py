@@ -93,6 +93,21 @@ class ShadingSystem:
     def attribute(self, name, value):
         check(lib().oslmi_shadingsys_attribute(self.h, name.encode(), str(value).encode()))
 
+    def name_transform(self, name, matrix, inverse=None):
+        """A named coordinate system of the renderer services: matrix is
+        the space-to-common 4x4 in Imath's row-vector layout (translation in
+        the last row); inverse, if given, is what get_inverse_matrix returns
+        (SimpleRenderer::name_transform, src/testshade/simplerend.cpp:464).
+        "shader" and "object" name the batch's shader2common and
+        object2common matrices."""
+        import numpy as np
+        m = np.ascontiguousarray(np.asarray(matrix, np.float32).reshape(16))
+        inv = None if inverse is None else np.ascontiguousarray(
+            np.asarray(inverse, np.float32).reshape(16))
+        check(lib().oslmi_renderer_name_transform(
+            self.h, name.encode(), m.ctypes.data_as(ctypes.c_void_p),
+            None if inv is None else inv.ctypes.data_as(ctypes.c_void_p)))
+
     def LoadMemoryCompiledShader(self, shadername, oso_text):
         check(lib().oslmi_load_memory_compiled_shader(self.h, shadername.encode(),
                                                       oso_text.encode()))

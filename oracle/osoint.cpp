@@ -32,6 +32,7 @@
 #include "color.h"
 #include "gabor.h"
 #include "noise.h"
+#include "omatrix.h"
 
 namespace oref {
 namespace oso {
@@ -237,12 +238,20 @@ struct Conn {
     int src_layer, src_sym, dst_layer, dst_sym;
 };
 
+// a renderer's named coordinate system (to-common matrix, optional inverse)
+struct NamedSpace {
+    Mat4 m, inv;
+    bool has_inv = false;
+};
+
 struct Group {
     std::vector<std::unique_ptr<Master>> masters;
     std::vector<Layer> layers;
     std::vector<Conn> conns;
     ColorSys cs;
     std::string fout = "fout", cout = "cout";  // last-layer outputs read by run
+    std::map<std::string, NamedSpace> spaces;
+    std::string commonspace = "world";
 };
 
 struct Exec {
@@ -273,6 +282,45 @@ struct Exec {
         return V3(v, v, v);
     }
     const std::string& Sv(int L, int s) const { return G.layers[L].m->syms[s].sdef.at(0); }
+
+    // renderer services over the named spaces (wide_opmatrix.cpp:533-610,
+    // opmatrix.cpp:232-262): unknown spaces give the identity and false
+    bool is_common(const std::string& n) const { return n == "common" || n == G.commonspace; }
+    bool space_matrix(const std::string& from, Mat4& M) const
+    {
+        M = mat4_identity();
+        if (is_common(from))
+            return true;
+        auto it = G.spaces.find(from);
+        if (it == G.spaces.end())
+            return false;
+        M = it->second.m;
+        return true;
+    }
+    bool space_inverse(const std::string& to, Mat4& M) const
+    {
+        M = mat4_identity();
+        if (is_common(to))
+            return true;
+        auto it = G.spaces.find(to);
+        if (it == G.spaces.end())
+            return false;
+        M = it->second.has_inv ? it->second.inv : mat4_inverse(it->second.m);
+        return true;
+    }
+    bool from_to(const std::string& from, const std::string& to, Mat4& M) const
+    {
+        if (is_common(from))
+            return space_inverse(to, M);
+        if (is_common(to))
+            return space_matrix(from, M);
+        Mat4 a, b;
+        const bool ok1 = space_matrix(from, a);
+        const bool ok2 = space_inverse(to, b);
+        M              = mat4_mul(a, b);
+        return ok1 && ok2;
+    }
+    std::string canon(const std::string& n) const { return n == G.commonspace ? "common" : n; }
     Base B(int L, int s) const { return G.layers[L].m->syms[s].base; }
     void setF(int L, int s, float v)
     {
@@ -571,20 +619,50 @@ struct Exec {
             return;
         }
         if (o == "matrix") {
-            for (int k = 0; k < 16; ++k)
-                vals[L][r].f[k] = F(L, a(k + 1));
+            if (op.args.size() == 17) {
+                for (int k = 0; k < 16; ++k)
+                    vals[L][r].f[k] = F(L, a(k + 1));
+                return;
+            }
+            // matrix(from) = from -> common; matrix(from, to)
+            const std::string from = canon(Sv(L, a(1)));
+            const std::string to   = op.args.size() == 3 ? canon(Sv(L, a(2))) : "common";
+            Mat4 M = mat4_identity();
+            if (from != to)
+                from_to(from, to, M);
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j)
+                    vals[L][r].f[4 * i + j] = M.x[i][j];
             return;
         }
-        if (o == "transform") {
-            const float* m = vals[L][a(1)].f;
-            const V3 p     = T(L, a(2));
-            const Sym& ps  = G.layers[L].m->syms[a(2)];
-            (void)ps;
-            const float x = p.x * m[0] + p.y * m[4] + p.z * m[8] + m[12];
-            const float y = p.x * m[1] + p.y * m[5] + p.z * m[9] + m[13];
-            const float z = p.x * m[2] + p.y * m[6] + p.z * m[10] + m[14];
-            const float w = p.x * m[3] + p.y * m[7] + p.z * m[11] + m[15];
-            setT(L, r, V3(x / w, y / w, z / w));
+        if (o == "getmatrix") {
+            const std::string from = canon(Sv(L, a(1))), to = canon(Sv(L, a(2)));
+            Mat4 M   = mat4_identity();
+            bool ok  = from == to || from_to(from, to, M);
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j)
+                    vals[L][a(3)].f[4 * i + j] = M.x[i][j];
+            vals[L][r].i = ok ? 1 : 0;
+            return;
+        }
+        if (o == "transform" || o == "transformv" || o == "transformn") {
+            // opcode = semantics (batched_llvm_gen.cpp:3485-3493); transform
+            // [from] to p by named spaces, or by a matrix
+            const int n  = (int)op.args.size();
+            const V3 p   = T(L, a(n == 3 ? 2 : 3));
+            Mat4 M;
+            if (n == 3 && B(L, a(1)) == kMatrix) {
+                M = mat4_from(vals[L][a(1)].f);
+            } else {
+                const std::string from = n == 4 ? canon(Sv(L, a(1))) : "common";
+                const std::string to   = canon(Sv(L, a(n == 3 ? 1 : 2)));
+                if (from == to || !from_to(from, to, M)) {  // identity, or unknown space
+                    setT(L, r, p);
+                    return;
+                }
+            }
+            setT(L, r, o == "transform" ? mat4_point(M, p)
+                                        : (o == "transformv" ? mat4_dir(M, p) : mat4_normal(M, p)));
             return;
         }
         if (o == "closure") {
@@ -668,6 +746,18 @@ void* oref_group_new(void)
     return g;
 }
 void oref_group_free(void* h) { delete (Group*)h; }
+
+// A named coordinate system: to-common matrix m (16 floats, row-vector
+// layout) and optional inverse (NULL: inverted like the default
+// get_inverse_matrix).
+void oref_group_name_transform(void* h, const char* name, const float* m, const float* inv)
+{
+    NamedSpace& sp = ((Group*)h)->spaces[name];
+    sp.m           = mat4_from(m);
+    sp.has_inv     = inv != nullptr;
+    if (inv)
+        sp.inv = mat4_from(inv);
+}
 
 // Adds a layer from OSO text; returns 0 or -1 (error text to stderr).
 int oref_group_layer(void* h, const char* oso, const char* layername)

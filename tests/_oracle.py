@@ -240,6 +240,17 @@ class Group:
         except Exception:
             pass
 
+    def name_transform(self, name, matrix, inverse=None):
+        """A named coordinate system (to-common matrix, optional inverse)."""
+        self.L.oref_group_name_transform.argtypes = [ctypes.c_void_p, ctypes.c_char_p,
+                                                     ctypes.c_void_p, ctypes.c_void_p]
+        m = np.ascontiguousarray(np.asarray(matrix, np.float32).reshape(16))
+        inv = None if inverse is None else np.ascontiguousarray(
+            np.asarray(inverse, np.float32).reshape(16))
+        self.L.oref_group_name_transform(self.h, name.encode(), _p(m),
+                                         None if inv is None else _p(inv))
+        return self
+
     def outputs(self, fname, cname):
         """Which last-layer float / triple outputs run() reports."""
         self.L.oref_group_outputs.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]
