@@ -79,9 +79,12 @@ typedef struct oslmi_batched_globals {
 oslmi_shadingsys* oslmi_shadingsys_create(const char* include_dir, const char* cache_dir);
 void oslmi_shadingsys_destroy(oslmi_shadingsys* ss);
 /* ShadingSystem::attribute (oslexec.h:177-324): "colorspace", "jit_cache",
- * "include_dir", and "gpu_waves_per_eu" (0..8: the occupancy, in waves per
- * SIMD, the emitted kernels are register-allocated for; default 6, 0 leaves
- * it to the compiler). */
+ * "include_dir", "commonspace" (the synonym of "common", default "world"),
+ * "closure_sample" (0/1: surface kernels compiled afterwards also carry
+ * CompositeBSDF::sample, for oslmi_execute_batch_sample), and
+ * "gpu_waves_per_eu" (0..8: the occupancy, in waves per SIMD, the emitted
+ * kernels are register-allocated for; default 6, 0 leaves it to the
+ * compiler). */
 int oslmi_shadingsys_attribute(oslmi_shadingsys* ss, const char* name, const char* value);
 /* Named coordinate systems, the renderer services behind transform(),
  * matrix(space) and getmatrix(): RendererServices::get_matrix /
@@ -134,6 +137,17 @@ double oslmi_group_jit_seconds(oslmi_group* g);
 int oslmi_execute_batch(oslmi_shadingsys* ss, oslmi_group* g, const oslmi_batched_globals* bg,
                         float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
                         void* stream);
+/* The same execution with the surface closure Ci sampled instead of
+ * evaluated, as the renderer's CompositeBSDF::sample(wo, rx, ry, rz)
+ * (src/testrender/shading.h:376-402, called from simpleraytracer.cpp's
+ * path loop): d_wo 3 planes, d_rnd 3 planes (rx, ry, rz in [0,1)),
+ * d_samples 7 planes (wi xyz, weight rgb, pdf; all zero when no lobe
+ * was drawn). The group must have been compiled after setting the
+ * shading-system attribute "closure_sample" to 1. */
+int oslmi_execute_batch_sample(oslmi_shadingsys* ss, oslmi_group* g,
+                               const oslmi_batched_globals* bg, float* d_outputs,
+                               const float* d_wo, const float* d_rnd, float* d_samples,
+                               void* stream);
 
 /* ---- Multi-group batches -----------------------------------------------
  * Many groups shaded in one persistent launch. The reference executes one

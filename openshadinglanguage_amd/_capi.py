@@ -44,6 +44,7 @@ def _declare(L):
         "oslmi_group_output_planes": ([P, P], I),
         "oslmi_group_jit_seconds": ([P], ctypes.c_double),
         "oslmi_execute_batch": ([P, P, P, P, P, P, P, P], I),
+        "oslmi_execute_batch_sample": ([P, P, P, P, P, P, P, P], I),
         "oslmi_batch_create": ([P, I, P, I, I], P),
         "oslmi_batch_destroy": ([P], None),
         "oslmi_batch_jit_seconds": ([P], ctypes.c_double),

@@ -14,7 +14,13 @@
 //   fresnel_dielectric         src/testrender/optics.h:13-31
 //   CompositeBSDF prepare/eval src/testrender/shading.h:323-372
 //   MIS::update_eval           src/testrender/sampling.h:173-206
+//   CompositeBSDF::sample      src/testrender/shading.h:376-402
+//   Diffuse<0>::sample         src/testrender/shading.cpp:313-320 with
+//     sample_cosine_hemisphere, to_unit_disk  sampling.h:70-100
+//   Microfacet<GGXDist,0>::sample  src/testrender/shading.cpp:655-685,
+//     sampleMicronormal :759-795, GGXDist::sampleSlope :483-507
 #pragma once
+#include "oslmi_math.h"
 #include "oslmi_types.h"
 
 namespace oslmi {
@@ -190,6 +196,163 @@ OSLMI_DEV void mis_update(V3* w, float* pdf, V3 ow, float opdf, float b)
         *w = *w * (1 - mis) + ow * mis;
         *pdf += opdf;
     }
+}
+
+// ---- sampling -------------------------------------------------------------
+struct BsdfDirSample {
+    V3 wi;
+    V3 weight;
+    float pdf;
+};
+
+// TangentFrame::get: x u + y v + z w
+OSLMI_DEV V3 frame_get(const Frame& f, float x, float y, float z)
+{
+    return f.u * x + f.v * y + f.w * z;
+}
+
+// concentric square-to-disk map; the sine lands in x, the cosine in y, as
+// testrender's OIIO::fast_sincos(phi, &x, &y) call has it
+OSLMI_DEV void to_unit_disk(float& x, float& y)
+{
+    float phi, r;
+    const float a = 2 * x - 1;
+    const float b = 2 * y - 1;
+    if (a * a > b * b) {
+        r   = a;
+        phi = 0.785398185253143310547f * (b / a);  // float(M_PI_4)
+    } else if (b != 0) {
+        r   = b;
+        phi = 1.57079637050628662109f - 0.785398185253143310547f * (a / b);
+    } else {
+        r   = 0;
+        phi = 0;
+    }
+    fsincos(phi, &x, &y);
+    x = x * r;
+    y = y * r;
+}
+
+OSLMI_DEV void ggx_sample_slope(float cos_theta, float u, float v, float* sx, float* sy)
+{
+    const float c   = cos_theta < 1e-6f ? 1e-6f : cos_theta;
+    const float Q   = (1 + c) * u - c;
+    const float num = c * sqrtf((1 - c) * (1 + c)) - Q * sqrtf((1 - Q) * (1 + Q));
+    float den       = (Q - c) * (Q + c);
+    const float eps = 1.0f / 4294967296.0f;
+    if (fabsf(den) < eps)
+        den = copysignf(eps, den);
+    *sx            = num / den;
+    const float Ru = 1 - 2 * v;
+    const float u2 = fabsf(Ru);
+    const float z  = (u2 * (u2 * (u2 * 0.27385f - 0.73369f) + 0.46341f))
+                    / (u2 * (u2 * (u2 * 0.093073f + 0.309420f) - 1.0f) + 0.597999f);
+    *sy = copysignf(1.0f, Ru) * z * sqrtf(1.0f + *sx * *sx);
+}
+
+OSLMI_DEV V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
+{
+    V3 swo(wo.x * xa, wo.y * ya, wo.z);
+    const float sl = sqrtf(dot(swo, swo));
+    if (sl != 0.0f)
+        swo = V3(swo.x / sl, swo.y / sl, swo.z / sl);
+    const float cos_theta = fmaxf(swo.z, 0.0f);
+    float cos_phi = 1, sin_phi = 0;
+    if (cos_theta < 0.99999f) {
+        const float inv = 1 / sqrtf(swo.x * swo.x + swo.y * swo.y);
+        cos_phi         = swo.x * inv;
+        sin_phi         = swo.y * inv;
+    }
+    float slx, sly;
+    ggx_sample_slope(cos_theta, u, v, &slx, &sly);
+    float tx         = cos_phi * slx - sin_phi * sly;
+    float ty         = sin_phi * slx + cos_phi * sly;
+    tx               = tx * xa;
+    ty               = ty * ya;
+    const float mlen = sqrtf(tx * tx + ty * ty + 1);
+    return V3(fabsf(tx) < mlen ? -tx / mlen : 1.0f, fabsf(ty) < mlen ? -ty / mlen : 1.0f,
+              1.0f / mlen);
+}
+
+OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
+{
+    BsdfDirSample r;
+    r.wi     = V3(0.0f);
+    r.weight = V3(0.0f);
+    r.pdf    = 0.0f;
+    if (l.id == kClosureDiffuse) {
+        to_unit_disk(rx, ry);
+        const float cos_theta = sqrtf(fmaxf(1 - rx * rx - ry * ry, 0.0f));
+        r.wi     = frame_get(frame_from_normal(l.N), rx, ry, cos_theta);
+        r.weight = V3(1.0f);
+        r.pdf    = cos_theta * 0.31830987334251403809f;
+        return r;
+    }
+    const Frame tf    = frame_from_normal(l.N);
+    const V3 wo_l     = tolocal(tf, wo);
+    const float cosNO = wo_l.z;
+    if (!(cosNO > 0))
+        return r;
+    const V3 m        = ggx_sample_micronormal(wo_l, l.alpha, l.alpha, rx, ry);
+    const float cosMO = dot(m, wo_l);
+    const float F     = fresnel_dielectric(cosMO, l.eta);
+    const V3 wi_l     = m * (2.0f * cosMO) - wo_l;
+    const float D     = ggx_D(m, l.alpha, l.alpha);
+    const float Lo    = ggx_lambda(wo_l, l.alpha, l.alpha);
+    const float Li    = ggx_lambda(wi_l, l.alpha, l.alpha);
+    const float G2    = 1 / (Lo + Li + 1);
+    const float G1    = 1 / (Lo + 1);
+    r.wi              = frame_get(tf, wi_l.x, wi_l.y, wi_l.z);
+    r.pdf             = (G1 * D * 0.25f) / cosNO;
+    r.weight          = V3(F * (G2 / G1));
+    return r;
+}
+
+template<int MAXL> OSLMI_DEV BsdfEval composite_eval(const LobeList<MAXL>& L, V3 wo, V3 wi);
+
+// CompositeBSDF::sample: rx picks a lobe by the prepared pdfs and is
+// rescaled for it; the other lobes are evaluated at the drawn direction and
+// MIS-combined in closure order. Compile-time lobe slots throughout.
+template<int MAXL>
+OSLMI_DEV BsdfDirSample composite_sample(const LobeList<MAXL>& L, V3 wo, float rx, float ry,
+                                         float rz)
+{
+    (void)rz;
+    BsdfDirSample s;
+    s.wi        = V3(0.0f);
+    s.weight    = V3(0.0f);
+    s.pdf       = 0.0f;
+    float accum = 0;
+    bool done   = false;
+#pragma unroll
+    for (int i = 0; i < MAXL; i++) {
+        if (!done && i < L.n) {
+            if (rx < (L.pdf[i] + accum)) {
+                done    = true;
+                float u = (rx - accum) / L.pdf[i];
+                u       = fminf(u, 0.99999994f);
+                s       = lobe_sample(L.l[i], wo, u, ry);
+                const V3 wsc = L.l[i].weight * (1 / L.pdf[i]);
+                s.weight     = s.weight * wsc;
+                s.pdf        = s.pdf * L.pdf[i];
+                if (s.pdf == 0.0f) {
+                    s.wi     = V3(0.0f);
+                    s.weight = V3(0.0f);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < MAXL; j++)
+                        if (j != i && j < L.n) {
+                            BsdfEval b = lobe_eval(L.l[j], wo, s.wi);
+                            mis_update(&s.weight, &s.pdf, b.weight * L.l[j].weight, b.pdf,
+                                       L.pdf[j]);
+                        }
+                }
+            } else {
+                accum += L.pdf[i];
+            }
+        }
+    }
+    return s;
 }
 
 template<int MAXL> OSLMI_DEV BsdfEval composite_eval(const LobeList<MAXL>& L, V3 wo, V3 wi)

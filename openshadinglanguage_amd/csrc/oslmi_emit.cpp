@@ -1191,7 +1191,7 @@ struct Emitter {
             k << "        Closure ci_;\n";
         k << carriers.str();
         k << b;
-        if (g.surface_closure) {
+        if (g.surface_closure && !ss.closure_sample) {
             k << "        if (A.eval) {\n"
                  "            V3 wo_(A.wo[i], A.wo[A.n + i], A.wo[2 * A.n + i]);\n"
                  "            V3 wi_(A.wi[i], A.wi[A.n + i], A.wi[2 * A.n + i]);\n"
@@ -1200,7 +1200,28 @@ struct Emitter {
                  "            A.eval[i] = e_.weight.x; A.eval[A.n + i] = e_.weight.y;\n"
                  "            A.eval[2 * A.n + i] = e_.weight.z; A.eval[3 * A.n + i] = e_.pdf;\n"
                  "        }\n";
+        } else if (g.surface_closure) {
+            // evaluation and/or CompositeBSDF::sample of Ci at wo
+            k << "        if (A.eval || A.samp) {\n"
+                 "            V3 wo_(A.wo[i], A.wo[A.n + i], A.wo[2 * A.n + i]);\n"
+                 "            composite_prepare(ci_, wo_);\n"
+                 "            if (A.eval) {\n"
+                 "                V3 wi_(A.wi[i], A.wi[A.n + i], A.wi[2 * A.n + i]);\n"
+                 "                BsdfEval e_ = composite_eval(ci_, wo_, wi_);\n"
+                 "                A.eval[i] = e_.weight.x; A.eval[A.n + i] = e_.weight.y;\n"
+                 "                A.eval[2 * A.n + i] = e_.weight.z; A.eval[3 * A.n + i] = e_.pdf;\n"
+                 "            }\n"
+                 "            if (A.samp) {\n"
+                 "                BsdfDirSample s_ = composite_sample(ci_, wo_, A.rnd[i], A.rnd[A.n + i],\n"
+                 "                                                   A.rnd[2 * A.n + i]);\n"
+                 "                A.samp[i] = s_.wi.x; A.samp[A.n + i] = s_.wi.y;\n"
+                 "                A.samp[2 * A.n + i] = s_.wi.z; A.samp[3 * A.n + i] = s_.weight.x;\n"
+                 "                A.samp[4 * A.n + i] = s_.weight.y; A.samp[5 * A.n + i] = s_.weight.z;\n"
+                 "                A.samp[6 * A.n + i] = s_.pdf;\n"
+                 "            }\n"
+                 "        }\n";
         }
+        g.can_sample    = g.surface_closure && ss.closure_sample;
         code            = k.str();
         g.uses_colorsys = uses_cs;
         g.globals_used.assign(globals.begin(), globals.end());

@@ -61,6 +61,7 @@ struct Group {
     bool surface_closure     = false;  // last layer writes Ci
     bool uses_colorsys       = false;
     bool uses_xforms         = false;  // reads the transform table
+    bool can_sample          = false;  // kernel samples Ci when A.samp is set
     std::vector<unsigned> globals_used;  // GlobalIndex values read
     hipModule_t module       = nullptr;
     hipFunction_t kernel     = nullptr;
@@ -101,6 +102,7 @@ struct ShadingSys {
     std::string cache_dir;   // code-object cache ("" = none)
     std::string include_dir; // device library headers for hiprtc
     int waves_per_eu = 6;    // occupancy target of emitted kernels (0: compiler default)
+    bool closure_sample = false;  // emit CompositeBSDF::sample into surface kernels
     mutable XformTable xf;
     std::mutex mu;
 };

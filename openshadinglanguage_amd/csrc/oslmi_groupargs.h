@@ -30,6 +30,10 @@ struct GroupArgs {
     float* eval;
     // the shading system's transform table (33 floats per space pair)
     const float* xf;
+    // optional closure sampling (kernels emitted with sampling): rnd 3
+    // planes (rx, ry, rz), samp 7 planes (wi xyz, weight rgb, pdf)
+    const float* rnd;
+    float* samp;
 };
 
 // One unit of work of the persistent batch kernel: points [begin, end) of

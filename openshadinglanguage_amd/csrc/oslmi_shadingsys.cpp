@@ -373,6 +373,13 @@ int oslmi_shadingsys_attribute(oslmi_shadingsys* s, const char* name, const char
         std::lock_guard<std::mutex> lock(s->ss.xf.mu);
         s->ss.commonspace = value;
         s->ss.xf.dirty    = true;
+    } else if (n == "closure_sample") {
+        // surface kernels JIT-compiled from now on also carry
+        // CompositeBSDF::sample (oslmi_execute_batch_sample)
+        std::string v(value);
+        if (v != "0" && v != "1")
+            return set_error(OSLMI_ERR_INVALID, "closure_sample must be 0 or 1");
+        s->ss.closure_sample = v == "1";
     } else if (n == "gpu_waves_per_eu") {
         // occupancy target (waves per SIMD) the emitted kernels are
         // register-allocated for; 0 leaves it to the compiler
@@ -596,6 +603,38 @@ int oslmi_execute_batch(oslmi_shadingsys* s, oslmi_group* gh, const oslmi_batche
                            "hipModuleLaunchKernel"))
         return rc;
     return OSLMI_OK;
+}
+
+int oslmi_execute_batch_sample(oslmi_shadingsys* s, oslmi_group* gh,
+                               const oslmi_batched_globals* bg, float* d_outputs,
+                               const float* d_wo, const float* d_rnd, float* d_samples,
+                               void* stream)
+{
+    if (!s || !gh || !bg || !d_wo || !d_rnd || !d_samples)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch_sample: null argument");
+    Group& g = *gh->g;
+    if (!g.jitted) {
+        if (int rc = oslmi_jit_group(s, gh))
+            return rc;
+    }
+    if (!g.can_sample)
+        return set_error(OSLMI_ERR_INVALID,
+                         "oslmi_execute_batch_sample: group " + g.name
+                             + " has no surface closure or was compiled without the"
+                               " closure_sample attribute");
+    GroupArgs A;
+    if (int rc = make_group_args(s->ss, g, bg, d_outputs, nullptr, nullptr, nullptr,
+                                 "oslmi_execute_batch_sample", A))
+        return rc;
+    A.wo   = d_wo;
+    A.rnd  = d_rnd;
+    A.samp = d_samples;
+    if (bg->n == 0)
+        return OSLMI_OK;
+    void* args[] = { &A };
+    return check_hip(hipModuleLaunchKernel(g.kernel, grid_for(bg->n), 1, 1, 256, 1, 1, 0,
+                                           (hipStream_t)stream, args, nullptr),
+                     "hipModuleLaunchKernel");
 }
 
 }  // extern "C"

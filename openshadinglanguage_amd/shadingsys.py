@@ -202,6 +202,20 @@ class ShadingSystem:
         return launch
 
 
+    def sample(self, group, n, globals_, wo, rnd, samples, outputs=None, stream=None):
+        """Batched execution with the surface closure sampled at wo
+        (CompositeBSDF::sample): rnd 3 planes (rx, ry, rz), samples 7
+        planes (wi xyz, weight rgb, pdf). Needs the group compiled after
+        attribute("closure_sample", 1)."""
+        bg = batched_globals(n, globals_)
+        import torch
+        sp = ctypes.c_void_p(stream.cuda_stream if stream is not None
+                             else torch.cuda.current_stream().cuda_stream)
+        check(lib().oslmi_execute_batch_sample(self.h, group.h, ctypes.byref(bg),
+                                               _ptr(outputs), _ptr(wo), _ptr(rnd),
+                                               _ptr(samples), sp))
+
+
 def build_fbm_cell(ss=None, outputs=("Cout",)):
     """The single-layer fbm_cell group."""
     ss = ss or ShadingSystem()

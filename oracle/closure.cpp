@@ -8,10 +8,19 @@
 //   fresnel_dielectric               src/testrender/optics.h:13-31
 //   CompositeBSDF prepare/eval       src/testrender/shading.h:323-372
 //   MIS::update_eval                 src/testrender/sampling.h:173-206
+//   CompositeBSDF::sample            src/testrender/shading.h:376-402
+//   Diffuse<0>::sample               src/testrender/shading.cpp:313-320,
+//     sample_cosine_hemisphere / to_unit_disk  sampling.h:70-100
+//   Microfacet<GGXDist,0>::sample    src/testrender/shading.cpp:655-685,
+//     sampleMicronormal :759-795, GGXDist::sampleSlope :483-507
+// to_unit_disk calls OIIO::fast_sincos, which is not in the reference tree;
+// the oracle's own sincos (omath.h) stands in (parity unpinned at ulp level).
 #include "closure.h"
 
 #include <algorithm>
 #include <cfloat>
+
+#include "omath.h"
 
 namespace oref {
 
@@ -105,6 +114,106 @@ BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
     return { V3(0.0f), 0.0f };
 }
 
+// TangentFrame::get: x u + y v + z w
+V3 frame_get(const Frame& f, float x, float y, float z) { return f.u * x + f.v * y + f.w * z; }
+
+// concentric square-to-disk map (Shirley and Chiu)
+void to_unit_disk(float& x, float& y)
+{
+    const float quarter_pi = float(M_PI_4);
+    const float half_pi    = float(M_PI_2);
+    float phi, r;
+    const float a = 2 * x - 1;
+    const float b = 2 * y - 1;
+    if (a * a > b * b) {
+        r   = a;
+        phi = quarter_pi * (b / a);
+    } else if (b != 0) {
+        r   = b;
+        phi = half_pi - quarter_pi * (a / b);
+    } else {
+        r   = 0;
+        phi = 0;
+    }
+    // OIIO::fast_sincos(phi, &x, &y): x takes the sine, y the cosine
+    ofast_sincosf(phi, &x, &y);
+    x = x * r;
+    y = y * r;
+}
+
+// GGX visible-slope sampling for a stretched view direction at cos_theta
+void ggx_sample_slope(float cos_theta, float u, float v, float* sx, float* sy)
+{
+    const float c = cos_theta < 1e-6f ? float(1e-6f) : cos_theta;
+    const float Q = (1 + c) * u - c;
+    const float num = c * sqrtf((1 - c) * (1 + c)) - Q * sqrtf((1 - Q) * (1 + Q));
+    float den       = (Q - c) * (Q + c);
+    const float eps = 1.0f / 4294967296.0f;
+    if (fabsf(den) < eps)
+        den = copysignf(eps, den);
+    *sx            = num / den;
+    const float Ru = 1 - 2 * v;
+    const float u2 = fabsf(Ru);
+    const float z  = (u2 * (u2 * (u2 * 0.27385f - 0.73369f) + 0.46341f))
+                    / (u2 * (u2 * (u2 * 0.093073f + 0.309420f) - 1.0f) + 0.597999f);
+    *sy = copysignf(1.0f, Ru) * z * sqrtf(1.0f + *sx * *sx);
+}
+
+V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
+{
+    V3 swo(wo.x * xa, wo.y * ya, wo.z);
+    swo                   = normalize(swo);
+    const float cos_theta = std::max<float>(swo.z, 0.0f);
+    float cos_phi = 1, sin_phi = 0;
+    if (cos_theta < 0.99999f) {
+        const float inv = 1 / sqrtf(SQR(swo.x) + SQR(swo.y));
+        cos_phi         = swo.x * inv;
+        sin_phi         = swo.y * inv;
+    }
+    float slx, sly;
+    ggx_sample_slope(cos_theta, u, v, &slx, &sly);
+    float tx         = cos_phi * slx - sin_phi * sly;
+    float ty         = sin_phi * slx + cos_phi * sly;
+    tx               = tx * xa;
+    ty               = ty * ya;
+    const float mlen = sqrtf(tx * tx + ty * ty + 1);
+    return V3(fabsf(tx) < mlen ? -tx / mlen : float(1.0f),
+              fabsf(ty) < mlen ? -ty / mlen : float(1.0f),
+              1.0f / mlen);
+}
+
+BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
+{
+    BsdfDirSample r = { V3(0.0f), V3(0.0f), 0.0f };
+    if (l.kind == kLobeDiffuse) {
+        to_unit_disk(rx, ry);
+        const float cos_theta = sqrtf(std::max<float>(1 - rx * rx - ry * ry, 0.0f));
+        r.wi     = frame_get(frame_from_normal(l.N), rx, ry, cos_theta);
+        r.weight = V3(1.0f);
+        r.pdf    = cos_theta * float(M_1_PI);
+        return r;
+    }
+    const Frame tf    = frame_from_normal_tangent(l.N, V3(0.0f));
+    const V3 wo_l     = tolocal(tf, wo);
+    const float cosNO = wo_l.z;
+    if (!(cosNO > 0))
+        return r;
+    const float xa = l.alpha, ya = l.alpha;
+    const V3 m           = ggx_sample_micronormal(wo_l, xa, ya, rx, ry);
+    const float cosMO    = dot(m, wo_l);
+    const float F        = fresnel_dielectric(cosMO, l.eta);
+    const V3 wi_l        = m * (2.0f * cosMO) - wo_l;
+    const float D        = ggx_D(m, xa, ya);
+    const float Lambda_o = ggx_lambda(wo_l, xa, ya);
+    const float Lambda_i = ggx_lambda(wi_l, xa, ya);
+    const float G2       = 1 / (Lambda_o + Lambda_i + 1);
+    const float G1       = 1 / (Lambda_o + 1);
+    r.wi                 = frame_get(tf, wi_l.x, wi_l.y, wi_l.z);
+    r.pdf                = (G1 * D * 0.25f) / cosNO;
+    r.weight             = V3(F * (G2 / G1));
+    return r;
+}
+
 V3 lobe_albedo(const Lobe& l, V3 wo)
 {
     if (l.kind == kLobeDiffuse)
@@ -161,6 +270,34 @@ void Composite::prepare(V3 wo)
     if (total > 0 || total > 1)
         for (int i = 0; i < n; i++)
             pdfs[i] /= total;
+}
+
+BsdfDirSample Composite::sample(V3 wo, float rx, float ry, float rz) const
+{
+    (void)rz;  // reflection-only lobes draw no third number
+    float accum = 0;
+    for (int i = 0; i < n; i++) {
+        if (rx < (pdfs[i] + accum)) {
+            rx = (rx - accum) / pdfs[i];
+            rx = std::min<float>(rx, 0.99999994f);
+            BsdfDirSample s = lobe_sample(lobes[i], wo, rx, ry);
+            const V3 wsc    = lobes[i].weight * (1 / pdfs[i]);
+            s.weight        = s.weight * wsc;
+            s.pdf           = s.pdf * pdfs[i];
+            if (s.pdf == 0.0f)
+                return { V3(0.0f), V3(0.0f), 0.0f };
+            for (int j = 0; j < n; j++) {
+                if (i != j) {
+                    BsdfSample b = lobe_eval(lobes[j], wo, s.wi);
+                    b.weight     = b.weight * lobes[j].weight;
+                    update_eval(&s.weight, &s.pdf, b.weight, b.pdf, pdfs[j]);
+                }
+            }
+            return s;
+        }
+        accum += pdfs[i];
+    }
+    return { V3(0.0f), V3(0.0f), 0.0f };
 }
 
 BsdfSample Composite::eval(V3 wo, V3 wi) const

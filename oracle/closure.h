@@ -20,6 +20,13 @@ struct BsdfSample {
     float pdf;
 };
 
+// a sampled direction with its MIS-combined weight and pdf
+struct BsdfDirSample {
+    V3 wi;
+    V3 weight;
+    float pdf;
+};
+
 // Flattened closure list + CompositeBSDF::prepare / eval
 // (src/testrender/shading.h:319-372, MIS::update_eval sampling.h:173-206)
 struct Composite {
@@ -33,6 +40,7 @@ struct Composite {
     }
     void prepare(V3 wo);
     BsdfSample eval(V3 wo, V3 wi) const;
+    BsdfDirSample sample(V3 wo, float rx, float ry, float rz) const;
 };
 
 float fresnel_dielectric(float cosi, float eta);  // src/testrender/optics.h:13-31

@@ -231,4 +231,32 @@ void oref_composite_eval(int nlobes, const float* lobes, const float* wo, const 
     }
 }
 
+// CompositeBSDF prepare + sample over one lobe list: wo 3 planes, rnd 3
+// planes (rx, ry, rz); out 7 planes (wi xyz, weight rgb, pdf).
+void oref_composite_sample(int nlobes, const float* lobes, const float* wo, const float* rnd,
+                           float* out, int64_t n)
+{
+    Composite c;
+    for (int k = 0; k < nlobes; ++k) {
+        const float* r = lobes + 9 * k;
+        Lobe l;
+        l.kind   = (int)r[0];
+        l.weight = V3(r[1], r[2], r[3]);
+        l.N      = V3(r[4], r[5], r[6]);
+        l.alpha  = r[7];
+        l.eta    = r[8];
+        c.add(l);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        Composite ci = c;
+        const V3 o(wo[i], wo[n + i], wo[2 * n + i]);
+        ci.prepare(o);
+        const BsdfDirSample s = ci.sample(o, rnd[i], rnd[n + i], rnd[2 * n + i]);
+        const float v[7]      = { s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z,
+                                  s.pdf };
+        for (int c7 = 0; c7 < 7; ++c7)
+            out[c7 * n + i] = v[c7];
+    }
+}
+
 }  // extern "C"

@@ -804,13 +804,11 @@ void oref_group_outputs(void* h, const char* fname, const char* cname)
     g->cout  = cname ? cname : "";
 }
 
-// Runs the group over n points. Globals P, N (3 planes each); outputs of
-// the last layer: fout (1 plane) and cout (3 planes) into out4; with wo/wi
-// non-null, the closure Ci evaluated at (wo, wi) into eval4 (R, G, B, pdf).
-void oref_group_run(void* h, const float* P, const float* N, const float* wo, const float* wi,
-                    int64_t n, float* out4, float* eval4, int nthreads)
+namespace {
+void run_group(Group* g, const float* P, const float* N, const float* wo, const float* wi,
+               const float* rnd, int64_t n, float* out4, float* eval4, float* samp7,
+               int nthreads)
 {
-    Group* g = (Group*)h;
     const int last = (int)g->layers.size() - 1;
     const int sf   = g->layers[last].m->find(g->fout);
     const int sc   = g->layers[last].m->find(g->cout);
@@ -825,16 +823,25 @@ void oref_group_run(void* h, const float* P, const float* N, const float* wo, co
             out4[n + i]     = c.x;
             out4[2 * n + i] = c.y;
             out4[3 * n + i] = c.z;
-            if (eval4) {
+            if (eval4 || samp7) {
                 Composite ci = ex.has_ci ? ex.ci : Composite();
                 const V3 o(wo[i], wo[n + i], wo[2 * n + i]);
-                const V3 w(wi[i], wi[n + i], wi[2 * n + i]);
                 ci.prepare(o);
-                BsdfSample s     = ci.eval(o, w);
-                eval4[i]         = s.weight.x;
-                eval4[n + i]     = s.weight.y;
-                eval4[2 * n + i] = s.weight.z;
-                eval4[3 * n + i] = s.pdf;
+                if (eval4) {
+                    const V3 w(wi[i], wi[n + i], wi[2 * n + i]);
+                    BsdfSample s     = ci.eval(o, w);
+                    eval4[i]         = s.weight.x;
+                    eval4[n + i]     = s.weight.y;
+                    eval4[2 * n + i] = s.weight.z;
+                    eval4[3 * n + i] = s.pdf;
+                }
+                if (samp7) {
+                    const BsdfDirSample s = ci.sample(o, rnd[i], rnd[n + i], rnd[2 * n + i]);
+                    const float v[7] = { s.wi.x,     s.wi.y,     s.wi.z, s.weight.x,
+                                         s.weight.y, s.weight.z, s.pdf };
+                    for (int k = 0; k < 7; ++k)
+                        samp7[k * n + i] = v[k];
+                }
             }
         }
     };
@@ -855,6 +862,24 @@ void oref_group_run(void* h, const float* P, const float* N, const float* wo, co
         });
     for (auto& t : th)
         t.join();
+}
+}  // namespace
+
+// Runs the group over n points. Globals P, N (3 planes each); outputs of
+// the last layer: fout (1 plane) and cout (3 planes) into out4; with wo/wi
+// non-null, the closure Ci evaluated at (wo, wi) into eval4 (R, G, B, pdf).
+void oref_group_run(void* h, const float* P, const float* N, const float* wo, const float* wi,
+                    int64_t n, float* out4, float* eval4, int nthreads)
+{
+    run_group((Group*)h, P, N, wo, wi, nullptr, n, out4, eval4, nullptr, nthreads);
+}
+
+// The same, with Ci sampled (CompositeBSDF::sample) at wo with the random
+// numbers rnd (3 planes: rx, ry, rz) into samp7 (wi xyz, weight rgb, pdf).
+void oref_group_run_sample(void* h, const float* P, const float* N, const float* wo,
+                           const float* rnd, int64_t n, float* out4, float* samp7, int nthreads)
+{
+    run_group((Group*)h, P, N, wo, nullptr, rnd, n, out4, nullptr, samp7, nthreads);
 }
 
 }  // extern "C"

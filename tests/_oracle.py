@@ -397,3 +397,31 @@ def sample_error_summary(samp, samp64):
     s = error_summary(np.asarray(samp)[keep], np.asarray(samp64)[keep])
     s["frac_flipped"] = float(flip.mean())
     return s
+
+
+def composite_sample(lobes, wo, rnd):
+    """CompositeBSDF prepare + sample of one lobe list (rows as
+    composite_eval) at n (wo, rnd) pairs. Returns 7 planes (wi xyz, weight
+    rgb, pdf)."""
+    L = lib()
+    L.oref_composite_sample.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    lo = np.ascontiguousarray(lobes, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (wo, rnd)]
+    n = a[0].shape[1]
+    out = np.zeros((7, n), np.float32)
+    L.oref_composite_sample(lo.shape[0], _p(lo), _p(a[0]), _p(a[1]), _p(out), n)
+    return out
+
+
+def group_run_sample(group, P, N, wo, rnd, nthreads=1):
+    """The interpreter group's outputs and Ci sampled at (wo, rnd)."""
+    L = group.L
+    L.oref_group_run_sample.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_void_p,
+                                                               ctypes.c_void_p, ctypes.c_int]
+    a = [np.ascontiguousarray(x, np.float32) for x in (P, N, wo, rnd)]
+    n = a[0].shape[1]
+    out = np.zeros((4, n), np.float32)
+    samp = np.zeros((7, n), np.float32)
+    L.oref_group_run_sample(group.h, *[_p(x) for x in a], n, _p(out), _p(samp), nthreads)
+    return out, samp
