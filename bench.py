@@ -44,7 +44,10 @@ WORKLOADS = {
 # FETCH_SIZE reads half the bytes of a plain 256 MiB copy on this GPU (x2
 # scale), WRITE_SIZE reads them exactly (x1); profiles/layered3_pmc.txt. The
 # layered3 figure includes the kernel's register-spill (scratch) traffic.
-MEASURED_TRAFFIC = {"layered3": 2 * 41121.9 * 1024 + 85283.5 * 1024, "fbm_cell": None}
+MEASURED_TRAFFIC = {"layered3": 2 * 41121.9 * 1024 + 85283.5 * 1024, "fbm_cell": None,
+                    # the layered closure, 2,097,152 points per launch
+                    # (profiles/layered_closure_pmc.txt)
+                    "closure": 2 * 107329.7 * 1024 + 262144.0 * 1024}
 
 
 def _ops_per_point(name):
@@ -128,7 +131,8 @@ def bench_closure(steps, warmup, cpu, cpu_passes):
            "ms_per_step": round(ms_per_step, 4),
            "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                         "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
-                        "frac": round(achieved / VALU_PEAK_OPS, 4), "traffic": None,
+                        "frac": round(achieved / VALU_PEAK_OPS, 4),
+                        "traffic": MEASURED_TRAFFIC["closure"],
                         "ops_per_point": round(ops, 1), "kernel_ms": round(kernel_ms, 4),
                         "hbm_algorithmic_GBps": round(CLOSURE_BYTES_PER_POINT * n
                                                       / (kernel_ms / 1e3) / 1e9, 2)}}
