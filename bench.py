@@ -25,9 +25,9 @@ sys.path.insert(0, ROOT)
 
 # MI355X (gfx950) 32-bit VALU peak: 256 CUs x 128 lanes/clk x 2.4 GHz =
 # 78.6 T lane-ops/s (the published 157.3 TFLOP/s FP32 vector figure counts an
-# FMA as two flops). Measured on the box: k_layered3 issues 1.24 VALU
-# wave-instructions per CU-clock (SQ_INSTS_VALU / GRBM_GUI_ACTIVE), which
-# only a 2-per-clock issue rate allows.
+# FMA as two flops). Measured on MI355X: the emitted layered3 kernel issues
+# 1.35 VALU wave-instructions per CU-clock (SQ_INSTS_VALU / GRBM_GUI_ACTIVE),
+# which only a 2-per-clock issue rate allows.
 VALU_PEAK_OPS = 256 * 128 * 2.4e9
 HBM_PEAK_BPS = 8.0e12
 
