@@ -100,6 +100,16 @@ int oslmi_shadingsys_attribute(oslmi_shadingsys* ss, const char* name, const cha
  * then waits for the device once to update the table. */
 int oslmi_renderer_name_transform(oslmi_shadingsys* ss, const char* name, const float* m,
                                   const float* inv);
+/* Renderer attributes behind getattribute(name, value)
+ * (BatchedRendererServices::get_attribute / get_attribute_uniform,
+ * batched_rendererservices.h:128-150): type "int", "float", "color"/"point"/"vector"/"normal" or
+ * "matrix"; value points at 1, 1, 3 or 16 values of that type. A kernel's
+ * getattribute succeeds (returns 1 and writes its value) only when the
+ * attribute exists with the value's base type; otherwise it returns 0 and
+ * leaves the value unchanged. Takes effect like
+ * oslmi_renderer_name_transform. */
+int oslmi_renderer_set_attribute(oslmi_shadingsys* ss, const char* name, const char* type,
+                                 const void* value);
 /* LoadMemoryCompiledShader: register a shader master from OSO text. */
 int oslmi_load_memory_compiled_shader(oslmi_shadingsys* ss, const char* shadername,
                                       const char* oso_text);

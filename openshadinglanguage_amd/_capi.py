@@ -30,6 +30,7 @@ def _declare(L):
         "oslmi_shadingsys_destroy": ([P], None),
         "oslmi_shadingsys_attribute": ([P, C, C], I),
         "oslmi_renderer_name_transform": ([P, C, P, P], I),
+        "oslmi_renderer_set_attribute": ([P, C, C, P], I),
         "oslmi_load_memory_compiled_shader": ([P, C, C], I),
         "oslmi_shader_group_begin": ([P, C], P),
         "oslmi_parameter": ([P, P, C, C, P, I], I),

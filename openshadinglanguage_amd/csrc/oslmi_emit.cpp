@@ -933,6 +933,41 @@ struct Emitter {
             fail("unsupported matrix constructor");
             return;
         }
+        if (o == "getattribute") {
+            // int getattribute(name, output value): the renderer's attribute
+            // table (RendererServices::get_attribute); the value is written
+            // only when the renderer has the name with the value's type
+            std::string name;
+            if (op.args.size() != 3 || !strval(L, a(1), name)) {
+                fail("getattribute supports (constant name, value) only");
+                return;
+            }
+            const OSymbol& dst = S(L, a(2));
+            const BaseType b   = dst.type.base;
+            if (dst.type.arraylen != 0 || (b != kInt && b != kFloat && b != kTriple && b != kMatrix)) {
+                fail("getattribute of " + name + ": value must be an int, float, triple or matrix");
+                return;
+            }
+            const int off = attr_slot(ss, name);
+            if (off < 0) {
+                fail("too many distinct renderer attributes (at most " +
+                     std::to_string(kMaxAttrSlots) + ")");
+                return;
+            }
+            g.uses_xforms = true;
+            const std::string slot = "(A.xf + " + std::to_string(off) + ")";
+            const Kind dk          = kind(L, a(2));
+            std::string v;
+            if (b == kInt) v = "__float_as_int(" + slot + "[0])";
+            else if (b == kFloat) v = slot + "[0]";
+            else if (b == kTriple) v = "V3(" + slot + "[0], " + slot + "[1], " + slot + "[2])";
+            else v = "xf_matrix(" + slot + ")";
+            if (dk == K_DF) v = "Df(" + v + ")";
+            if (dk == K_DV) v = "Dv(" + v + ")";
+            line("if (" + slot + "[16] == " + std::to_string((int)b + 1) + ".0f) { " + var(L, a(2))
+                 + " = " + v + "; " + R + " = 1; } else { " + R + " = 0; }");
+            return;
+        }
         if (o == "getmatrix") {
             // int getmatrix(from, to, output M) (osl_get_from_to_matrix)
             std::string from, to;

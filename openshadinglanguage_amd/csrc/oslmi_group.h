@@ -60,7 +60,7 @@ struct Group {
     int output_planes        = 0;
     bool surface_closure     = false;  // last layer writes Ci
     bool uses_colorsys       = false;
-    bool uses_xforms         = false;  // reads the transform table
+    bool uses_xforms         = false;  // reads the transform / attribute table
     bool can_sample          = false;  // kernel samples Ci when A.samp is set
     std::vector<unsigned> globals_used;  // GlobalIndex values read
     hipModule_t module       = nullptr;
@@ -86,12 +86,27 @@ struct NamedXform {
 constexpr int kXformSlotFloats = 33;
 constexpr int kMaxXformSlots   = 256;
 
+// A renderer attribute (RendererServices::get_attribute): base type
+// kInt / kFloat / kTriple / kMatrix and its value (an int in v[0]'s bits).
+struct RendererAttr {
+    int base = 0;
+    float v[16] = {};
+};
+// Attribute table: one slot per attribute name some kernel reads,
+// kAttrSlotFloats floats each: the value (16), the base type + 1 when the
+// renderer has the attribute, else 0 (1).
+constexpr int kAttrSlotFloats = 17;
+constexpr int kMaxAttrSlots   = 256;
+
 struct XformTable {
     std::mutex mu;
     std::map<std::string, NamedXform> named;
     std::vector<std::pair<std::string, std::string>> pairs;
     float* device = nullptr;  // kMaxXformSlots slots, allocated on first use
     bool dirty    = true;     // device copy out of date
+    // renderer attributes, in the same table after the transform slots
+    std::map<std::string, RendererAttr> attrs;
+    std::vector<std::string> attr_names;
 };
 
 struct ShadingSys {
@@ -110,6 +125,10 @@ struct ShadingSys {
 // Slot of the (from, to) pair in the shading system's transform table,
 // added on first use; -1 when the table is full.
 int xform_slot(const ShadingSys& ss, const std::string& from, const std::string& to);
+
+// Slot of a renderer attribute name in the same table (offset in floats from
+// the table start), added on first use; -1 when the table is full.
+int attr_slot(const ShadingSys& ss, const std::string& name);
 
 // Brings the device copy of the transform table up to date (allocating it
 // on first use); a no-op when nothing changed.

@@ -255,6 +255,22 @@ int xform_slot(const ShadingSys& ss, const std::string& from, const std::string&
     return (int)ss.xf.pairs.size() - 1;
 }
 
+int attr_slot(const ShadingSys& ss, const std::string& name)
+{
+    std::lock_guard<std::mutex> lock(ss.xf.mu);
+    size_t k = 0;
+    for (; k < ss.xf.attr_names.size(); ++k)
+        if (ss.xf.attr_names[k] == name)
+            break;
+    if (k == ss.xf.attr_names.size()) {
+        if ((int)k >= kMaxAttrSlots)
+            return -1;
+        ss.xf.attr_names.push_back(name);
+        ss.xf.dirty = true;
+    }
+    return kXformSlotFloats * kMaxXformSlots + kAttrSlotFloats * (int)k;
+}
+
 int xform_table_sync(const ShadingSys& ss)
 {
     std::lock_guard<std::mutex> lock(ss.xf.mu);
@@ -262,7 +278,8 @@ int xform_table_sync(const ShadingSys& ss)
         return OSLMI_OK;
     if (!ss.xf.device) {
         if (int rc = check_hip(hipMalloc((void**)&ss.xf.device,
-                                         sizeof(float) * kXformSlotFloats * kMaxXformSlots),
+                                         sizeof(float) * (kXformSlotFloats * kMaxXformSlots
+                                                          + kAttrSlotFloats * kMaxAttrSlots)),
                                "hipMalloc(transform table)"))
             return rc;
     }
@@ -283,6 +300,22 @@ int xform_table_sync(const ShadingSys& ss)
         if (int rc = check_hip(hipMemcpy(ss.xf.device, host.data(), sizeof(float) * host.size(),
                                          hipMemcpyHostToDevice),
                                "hipMemcpy(transform table)"))
+            return rc;
+    }
+    std::vector<float> attrs(kAttrSlotFloats * ss.xf.attr_names.size(), 0.0f);
+    for (size_t k = 0; k < ss.xf.attr_names.size(); ++k) {
+        auto it = ss.xf.attrs.find(ss.xf.attr_names[k]);
+        if (it == ss.xf.attrs.end())
+            continue;
+        float* slot = attrs.data() + kAttrSlotFloats * k;
+        memcpy(slot, it->second.v, sizeof(it->second.v));
+        slot[16] = (float)(it->second.base + 1);
+    }
+    if (!attrs.empty()) {
+        if (int rc = check_hip(hipMemcpy(ss.xf.device + kXformSlotFloats * kMaxXformSlots,
+                                         attrs.data(), sizeof(float) * attrs.size(),
+                                         hipMemcpyHostToDevice),
+                               "hipMemcpy(attribute table)"))
             return rc;
     }
     ss.xf.dirty = false;
@@ -410,6 +443,25 @@ int oslmi_renderer_name_transform(oslmi_shadingsys* s, const char* name, const f
     if (inv)
         memcpy(x.inv, inv, sizeof(x.inv));
     s->ss.xf.dirty = true;
+    return OSLMI_OK;
+}
+
+int oslmi_renderer_set_attribute(oslmi_shadingsys* s, const char* name, const char* type,
+                                 const void* value)
+{
+    if (!s || !name || !type || !value || !*name)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_renderer_set_attribute: null argument");
+    OType t;
+    if (!parse_type(type, t) || t.base == kString)
+        return set_error(OSLMI_ERR_UNSUPPORTED, std::string("oslmi_renderer_set_attribute: type ")
+                                                    + type + " is not supported");
+    RendererAttr a;
+    a.base = t.base;
+    const int count = t.base == kMatrix ? 16 : (t.base == kTriple ? 3 : 1);
+    memcpy(a.v, value, sizeof(float) * count);  // an int keeps its bits in v[0]
+    std::lock_guard<std::mutex> lock(s->ss.xf.mu);
+    s->ss.xf.attrs[name] = a;
+    s->ss.xf.dirty       = true;
     return OSLMI_OK;
 }
 

@@ -108,6 +108,15 @@ class ShadingSystem:
             self.h, name.encode(), m.ctypes.data_as(ctypes.c_void_p),
             None if inv is None else inv.ctypes.data_as(ctypes.c_void_p)))
 
+    def set_attribute(self, name, typ, value):
+        """A renderer attribute for getattribute(): typ "int", "float", a
+        triple type or "matrix"."""
+        import numpy as np
+        dt = np.int32 if typ == "int" else np.float32
+        v = np.ascontiguousarray(np.asarray(value, dt).reshape(-1))
+        check(lib().oslmi_renderer_set_attribute(self.h, name.encode(), typ.encode(),
+                                                 v.ctypes.data_as(ctypes.c_void_p)))
+
     def LoadMemoryCompiledShader(self, shadername, oso_text):
         check(lib().oslmi_load_memory_compiled_shader(self.h, shadername.encode(),
                                                       oso_text.encode()))

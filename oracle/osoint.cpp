@@ -252,6 +252,12 @@ struct Group {
     std::string fout = "fout", cout = "cout";  // last-layer outputs read by run
     std::map<std::string, NamedSpace> spaces;
     std::string commonspace = "world";
+    // renderer attributes: base type and value (an int as v[0]'s bits)
+    struct Attr {
+        Base base;
+        float v[16];
+    };
+    std::map<std::string, Attr> attrs;
 };
 
 struct Exec {
@@ -635,6 +641,23 @@ struct Exec {
                     vals[L][r].f[4 * i + j] = M.x[i][j];
             return;
         }
+        if (o == "getattribute") {
+            // RendererServices::get_attribute: found with the value's type
+            auto it = G.attrs.find(Sv(L, a(1)));
+            const Base want = B(L, a(2));
+            if (it == G.attrs.end() || it->second.base != want) {
+                vals[L][r].i = 0;
+                return;
+            }
+            Val& d = vals[L][a(2)];
+            if (want == kInt)
+                std::memcpy(&d.i, &it->second.v[0], 4);
+            else
+                for (int k = 0; k < (want == kMatrix ? 16 : (want == kTriple ? 3 : 1)); ++k)
+                    d.f[k] = it->second.v[k];
+            vals[L][r].i = 1;
+            return;
+        }
         if (o == "getmatrix") {
             const std::string from = canon(Sv(L, a(1))), to = canon(Sv(L, a(2)));
             Mat4 M   = mat4_identity();
@@ -750,6 +773,18 @@ void oref_group_free(void* h) { delete (Group*)h; }
 // A named coordinate system: to-common matrix m (16 floats, row-vector
 // layout) and optional inverse (NULL: inverted like the default
 // get_inverse_matrix).
+// A renderer attribute: base 0 int, 1 float, 2 triple, 3 matrix (this
+// interpreter's Base order); value 1, 1, 3 or 16 values.
+void oref_group_set_attribute(void* h, const char* name, int base, const void* value)
+{
+    Group::Attr a;
+    a.base = (Base)base;
+    std::memset(a.v, 0, sizeof(a.v));
+    const int count = base == kMatrix ? 16 : (base == kTriple ? 3 : 1);
+    std::memcpy(a.v, value, 4 * count);
+    ((Group*)h)->attrs[name] = a;
+}
+
 void oref_group_name_transform(void* h, const char* name, const float* m, const float* inv)
 {
     NamedSpace& sp = ((Group*)h)->spaces[name];

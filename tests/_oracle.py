@@ -251,6 +251,16 @@ class Group:
                                          None if inv is None else _p(inv))
         return self
 
+    def set_attribute(self, name, typ, value):
+        """A renderer attribute for getattribute()."""
+        self.L.oref_group_set_attribute.argtypes = [ctypes.c_void_p, ctypes.c_char_p,
+                                                    ctypes.c_int, ctypes.c_void_p]
+        base = {"int": 0, "float": 1, "matrix": 3}.get(typ, 2)
+        v = np.ascontiguousarray(np.asarray(value, np.int32 if typ == "int" else np.float32)
+                                 .reshape(-1))
+        self.L.oref_group_set_attribute(self.h, name.encode(), base, _p(v))
+        return self
+
     def outputs(self, fname, cname):
         """Which last-layer float / triple outputs run() reports."""
         self.L.oref_group_outputs.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]
