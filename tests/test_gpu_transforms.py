@@ -107,3 +107,40 @@ def test_transform_derivatives_closed_form(dev, to):
         dh = np.asarray(d, np.float64) @ Minv[:3, :]  # derivative of (a, b, c, w)
         want = ((dh[None, :3] * h[:, 3:4] - h[:, :3] * dh[None, 3:4]) / h[:, 3:4] ** 2).T
         np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-6, err_msg=which)
+
+
+def test_transforms_and_attributes_in_a_batch(dev):
+    """The same groups in one persistent batch launch (oslmi_batch_*) read
+    the shading system's table through each item's GroupArgs: outputs are
+    bit-identical to the groups' own kernels."""
+    from test_attributes import ATTRS, attr_shader
+    sp = spaces()
+    ss = S.ShadingSystem()
+    for name, m in sp.items():
+        ss.name_transform(name, m)
+    for name, typ, v in ATTRS:
+        ss.set_attribute(name, typ, v)
+    osos = [transform_shader(*c) for c in CASES[::3]] + [attr_shader()]
+    groups = [S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), "b_" + o.name)
+              for o in osos]
+    rng = np.random.default_rng(4)
+    items, outs, alone = [], [], []
+    for k, g in enumerate(groups):
+        n = 300 + 50 * k
+        P = torch.from_numpy(rng.uniform(-3, 3, (3, n)).astype(np.float32)).to(dev)
+        glob = {"P": P, "N": P, "dPdx": (0.0, 0.0, 0.0), "dPdy": (0.0, 0.0, 0.0)}
+        o = torch.empty((4, n), device=dev)
+        items.append((n, glob, o, None, None, None))
+        outs.append(o)
+        ss.jit_group(g)
+        a = torch.empty((4, n), device=dev)
+        ss.execute(g, n, glob, outputs=a)
+        alone.append(a)
+    print("transform batch: %d groups" % len(groups), flush=True)
+    b = S.GroupBatch(ss, groups, groups_per_kernel=4)
+    b.bind(items)
+    b.launch()
+    torch.cuda.synchronize()
+    for k in range(len(groups)):
+        np.testing.assert_array_equal(outs[k].cpu().numpy(), alone[k].cpu().numpy(),
+                                      err_msg=osos[k].name)
