@@ -16,6 +16,7 @@
 //   * string arguments (noise names, closure names, color spaces, Gabor
 //     options) are compile-time constants, as in the reference's
 //     llvm_gen_noise name mapping (batched_llvm_gen.cpp:6096-6134).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <functional>
@@ -1012,6 +1013,22 @@ struct Emitter {
                 numval(L, a(8), refr);
                 if (!refr.empty() && refr[0] != 0.0f) {
                     fail("refracting microfacet is not supported");
+                    return;
+                }
+                // isotropic lobes about N only: yalpha must be xalpha (the
+                // same symbol, as oslc writes microfacet(dist, N, alpha,
+                // eta, refract), or equal constants) and U the constant 0
+                auto is_const = [&](int sym) { return S(L, sym).kind == kConst; };
+                std::vector<float> xa, ya, u;
+                const bool same_alpha =
+                    a(5) == a(6)
+                    || (is_const(a(5)) && is_const(a(6)) && numval(L, a(5), xa)
+                        && numval(L, a(6), ya) && xa == ya);
+                const bool zero_u = is_const(a(4)) && numval(L, a(4), u)
+                                    && std::all_of(u.begin(), u.end(),
+                                                   [](float v) { return v == 0.0f; });
+                if (!same_alpha || !zero_u) {
+                    fail("anisotropic microfacet (yalpha != xalpha or a tangent U) is not supported");
                     return;
                 }
                 line(R + " = closure_microfacet_ggx(" + ex(L, a(3), K_V3) + ", " + ex(L, a(5), K_FLOAT)

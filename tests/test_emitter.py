@@ -128,3 +128,30 @@ def test_batch_rejects_open_group():
     ss.Shader(g, "surface", "fbm_cell", "a")
     with pytest.raises(RuntimeError):
         S.GroupBatch(ss, [g], load=False)
+
+
+def _microfacet_shader(xa, ya, u):
+    from openshadinglanguage_amd import groupgen as G
+    o = G.Oso("surface", "mf_%s_%s_%s" % (xa, ya, u))
+    o.sym("global", "normal", "N")
+    o.sym("global", "closure color", "Ci")
+    o.sym("oparam", "float", "fout", "0")
+    o.sym("oparam", "color", "cout", "0 0 0")
+    cl = o.tmp("closure color")
+    o.op("closure", cl, o.const("string", "microfacet"), o.const("string", "ggx"), "N",
+         o.const("vector", *u), o.const("float", xa), o.const("float", ya),
+         o.const("float", 1.5), o.const("int", 0))
+    o.op("assign", "Ci", cl)
+    return o
+
+
+def test_anisotropic_microfacet_is_rejected():
+    ss = S.ShadingSystem()
+    ok = _microfacet_shader(0.3, 0.3, (0, 0, 0))
+    g = S.build_generated_group(ss, ([(ok.name, "l0", ok.text())], []), "mf_ok")
+    assert "closure_microfacet_ggx" in ss.emit(g)
+    for xa, ya, u in ((0.3, 0.2, (0, 0, 0)), (0.3, 0.3, (1, 0, 0))):
+        o = _microfacet_shader(xa, ya, u)
+        g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), "mf_bad")
+        with pytest.raises(OSLMIError):
+            ss.emit(g)
