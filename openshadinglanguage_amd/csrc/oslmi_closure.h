@@ -10,6 +10,8 @@
 //   Diffuse<0>                 src/testrender/shading.cpp:301-312
 //   Microfacet<GGXDist,0>      src/testrender/shading.cpp:572-615, :713-757
 //   GGXDist                    src/testrender/shading.cpp:497-506
+//   BeckmannDist F, Lambda     src/testrender/shading.cpp:510-522 (eval only;
+//     OIIO::fast_exp there, the library's own exp here)
 //   TangentFrame::from_normal  src/testrender/sampling.h:17-42
 //   fresnel_dielectric         src/testrender/optics.h:13-31
 //   CompositeBSDF prepare/eval src/testrender/shading.h:323-372
@@ -26,7 +28,7 @@
 namespace oslmi {
 
 // Closure ids as registered by testrender (shading.cpp:185-296 order).
-enum ClosureId { kClosureDiffuse = 1, kClosureMicrofacetGGX = 2 };
+enum ClosureId { kClosureDiffuse = 1, kClosureMicrofacetGGX = 2, kClosureMicrofacetBeckmann = 3 };
 
 struct Lobe {
     int id;
@@ -127,6 +129,32 @@ OSLMI_DEV float ggx_D(V3 Hr, float xa, float ya)
     return 0;
 }
 
+// Microfacet<BeckmannDist,0>: the same evalLambda / evalD with Beckmann's
+// slope distribution and Lambda approximation
+OSLMI_DEV float beckmann_lambda(V3 w, float xa, float ya)
+{
+    float cosTheta2  = w.z * w.z;
+    float cosPhi2st2 = (w.x * xa) * (w.x * xa);
+    float sinPhi2st2 = (w.y * ya) * (w.y * ya);
+    float a2         = cosTheta2 / (cosPhi2st2 + sinPhi2st2);
+    float a          = sqrtf(a2);
+    return a < 1.6f ? (1.0f - 1.259f * a + 0.396f * a2) / (3.535f * a + 2.181f * a2) : 0.0f;
+}
+OSLMI_DEV float beckmann_D(V3 Hr, float xa, float ya)
+{
+    float cosThetaM = Hr.z;
+    if (cosThetaM > 0) {
+        float cosPhi2st2 = (Hr.x / xa) * (Hr.x / xa);
+        float sinPhi2st2 = (Hr.y / ya) * (Hr.y / ya);
+        float cosThetaM2 = cosThetaM * cosThetaM;
+        float cosThetaM4 = cosThetaM2 * cosThetaM2;
+        float tanThetaM2 = (cosPhi2st2 + sinPhi2st2) / cosThetaM2;
+        float F          = 0.31830987334251403809f * fexp(-tanThetaM2);  // float(1 / M_PI)
+        return F / (xa * ya * cosThetaM4);
+    }
+    return 0;
+}
+
 OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
 {
     BsdfEval r;
@@ -142,9 +170,10 @@ OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
         V3 h     = wi_l + wo_l;
         float hl = sqrtf(dot(h, h));
         V3 m     = hl != 0.0f ? V3(h.x / hl, h.y / hl, h.z / hl) : h;
-        float D  = ggx_D(m, l.alpha, l.alpha);
-        float Lo = ggx_lambda(wo_l, l.alpha, l.alpha);
-        float Li = ggx_lambda(wi_l, l.alpha, l.alpha);
+        const bool bk = l.id == kClosureMicrofacetBeckmann;
+        float D  = bk ? beckmann_D(m, l.alpha, l.alpha) : ggx_D(m, l.alpha, l.alpha);
+        float Lo = bk ? beckmann_lambda(wo_l, l.alpha, l.alpha) : ggx_lambda(wo_l, l.alpha, l.alpha);
+        float Li = bk ? beckmann_lambda(wi_l, l.alpha, l.alpha) : ggx_lambda(wi_l, l.alpha, l.alpha);
         float G2 = 1 / (Lo + Li + 1);
         float G1 = 1 / (Lo + 1);
         float Fr = fresnel_dielectric(dot(m, wo_l), l.eta);

@@ -1000,8 +1000,12 @@ struct Emitter {
             } else if (cname == "microfacet") {
                 std::string dist;
                 strval(L, a(2), dist);
-                if (dist != "ggx") {
+                if (dist != "ggx" && dist != "beckmann") {
                     fail("microfacet distribution '" + dist + "' is not supported");
+                    return;
+                }
+                if (dist == "beckmann" && ss.closure_sample) {
+                    fail("sampling a beckmann microfacet is not supported");
                     return;
                 }
                 // microfacet(dist, N, U, xalpha, yalpha, eta, refract)
@@ -1031,7 +1035,7 @@ struct Emitter {
                     fail("anisotropic microfacet (yalpha != xalpha or a tangent U) is not supported");
                     return;
                 }
-                line(R + " = closure_microfacet_ggx(" + ex(L, a(3), K_V3) + ", " + ex(L, a(5), K_FLOAT)
+                line(R + " = closure_microfacet_" + dist + "(" + ex(L, a(3), K_V3) + ", " + ex(L, a(5), K_FLOAT)
                      + ", " + ex(L, a(7), K_FLOAT) + ");");
             } else {
                 fail("closure '" + cname + "' is not supported");

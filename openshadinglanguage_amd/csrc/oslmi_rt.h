@@ -209,6 +209,14 @@ OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
     c.add(l);
     return c;
 }
+OSLMI_DEV LobeList<1> closure_microfacet_beckmann(V3 N, float alpha, float eta)
+{
+    LobeList<1> c;
+    Lobe l;
+    l.id = kClosureMicrofacetBeckmann; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
+    c.add(l);
+    return c;
+}
 // color * closure (a MUL node): scales every lobe weight
 template<int K> OSLMI_DEV LobeList<K> mul(const LobeList<K>& c, V3 w)
 {

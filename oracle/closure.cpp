@@ -89,6 +89,26 @@ float ggx_D(V3 Hr, float xa, float ya)
     return 0;
 }
 
+// BeckmannDist (src/testrender/shading.cpp:510-522): F = exp(-tan^2) / pi
+// with the oracle's exp for OIIO::fast_exp, and Walter et al.'s rational
+// Lambda
+float beckmann_lambda(V3 w, float xa, float ya)
+{
+    const float a2 = SQR(w.z) / (SQR(w.x * xa) + SQR(w.y * ya));
+    const float a  = sqrtf(a2);
+    return a < 1.6f ? (1.0f - 1.259f * a + 0.396f * a2) / (3.535f * a + 2.181f * a2)
+                    : float(0.0f);
+}
+float beckmann_D(V3 Hr, float xa, float ya)
+{
+    const float c = Hr.z;
+    if (!(c > 0))
+        return 0;
+    const float c2   = SQR(c);
+    const float tan2 = (SQR(Hr.x / xa) + SQR(Hr.y / ya)) / c2;
+    return (float(1 / M_PI) * ofast_expf(-tan2)) / (xa * ya * SQR(c2));
+}
+
 BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
 {
     if (l.kind == kLobeDiffuse) {
@@ -101,9 +121,10 @@ BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
     const float xa  = l.alpha, ya = l.alpha;
     if (wo_l.z > 0 && wi_l.z > 0) {
         const V3 m           = normalize(wi_l + wo_l);
-        const float D        = ggx_D(m, xa, ya);
-        const float Lambda_o = ggx_lambda(wo_l, xa, ya);
-        const float Lambda_i = ggx_lambda(wi_l, xa, ya);
+        const bool bk        = l.kind == kLobeBeckmannRefl;
+        const float D        = bk ? beckmann_D(m, xa, ya) : ggx_D(m, xa, ya);
+        const float Lambda_o = bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya);
+        const float Lambda_i = bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya);
         const float G2       = 1 / (Lambda_o + Lambda_i + 1);
         const float G1       = 1 / (Lambda_o + 1);
         const float Fr       = fresnel_dielectric(dot(m, wo_l), l.eta);
@@ -193,6 +214,8 @@ BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
         r.pdf    = cos_theta * float(M_1_PI);
         return r;
     }
+    if (l.kind == kLobeBeckmannRefl)
+        return r;  // Beckmann lobes are evaluated, not sampled
     const Frame tf    = frame_from_normal_tangent(l.N, V3(0.0f));
     const V3 wo_l     = tolocal(tf, wo);
     const float cosNO = wo_l.z;

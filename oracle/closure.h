@@ -5,7 +5,7 @@
 
 namespace oref {
 
-enum LobeKind { kLobeDiffuse = 0, kLobeGGXRefl = 1 };
+enum LobeKind { kLobeDiffuse = 0, kLobeGGXRefl = 1, kLobeBeckmannRefl = 2 };
 
 struct Lobe {
     int kind;

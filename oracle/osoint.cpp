@@ -699,7 +699,7 @@ struct Exec {
                 l.alpha = 0.0f;
                 l.eta   = 0.0f;
             } else {
-                l.kind  = kLobeGGXRefl;
+                l.kind  = Sv(L, a(2)) == "beckmann" ? kLobeBeckmannRefl : kLobeGGXRefl;
                 l.N     = T(L, a(3));
                 l.alpha = F(L, a(5));
                 l.eta   = F(L, a(7));
@@ -750,6 +750,15 @@ struct Exec {
         for (size_t L = 0; L < G.layers.size(); ++L) {
             init_layer((int)L);
             block((int)L, 0, (int)G.layers[L].m->ops.size());
+            // Ci written by any closure op (add, mul, closure), not only
+            // by an assign: its final value is the layer's closure
+            const Master& m = *G.layers[L].m;
+            for (size_t k = 0; k < m.syms.size(); ++k)
+                if (m.syms[k].name == "Ci" && m.syms[k].kind == "global"
+                    && m.syms[k].base == kClosure && vals[L][k].cl.n > 0) {
+                    has_ci = true;
+                    ci     = vals[L][k].cl;
+                }
         }
     }
 };

@@ -90,3 +90,24 @@ def test_sampling_needs_the_attribute(dev):
     w = G.group_inputs(64, 1)
     with pytest.raises(OSLMIError):
         _run(dev, ss, g, w, np.zeros((3, 64), np.float32))
+
+
+def test_beckmann_eval_matches_interpreter(dev):
+    from test_emitter import _beckmann_shader
+    o = _beckmann_shader()
+    ss = S.ShadingSystem()
+    g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), "bk")
+    n = 4000
+    w = G.group_inputs(n, 77)
+    t = {k: torch.from_numpy(np.ascontiguousarray(w[k])).to(dev) for k in ("P", "N", "wo", "wi")}
+    out = torch.empty((4, n), device=dev)
+    ev = torch.empty((4, n), device=dev)
+    print("beckmann group: n=%d" % n, flush=True)
+    ss.execute(g, n, {"P": t["P"], "N": t["N"], "dPdx": (0.0, 0.0, 0.0), "dPdy": (0.0, 0.0, 0.0)},
+               outputs=out, wo=t["wo"], wi=t["wi"], eval_out=ev)
+    torch.cuda.synchronize()
+    ref_out, ref_ev = O.Group([(o.name, "l0", o.text())], []).run(w["P"], w["N"], w["wo"],
+                                                                  w["wi"], nthreads=8)
+    got = ev.cpu().numpy()
+    assert close(got, ref_ev).all()
+    assert float(np.mean(got == ref_ev)) > 0.999
