@@ -96,8 +96,10 @@ int oslmi_shadingsys_attribute(oslmi_shadingsys* ss, const char* name, const cha
  * (translation in m[12..14]); inv: its inverse, or NULL to have the library
  * invert m as OSL's default get_inverse_matrix does. "common" and the
  * commonspace synonym ("world") are the identity and cannot be named.
- * Takes effect at the next oslmi_execute_batch / oslmi_batch_bind, which
- * then waits for the device once to update the table. */
+ * Takes effect at the next oslmi_execute_batch, oslmi_batch_bind,
+ * oslmi_batch_launch or oslmi_batch_capture (also for a batch bound or a
+ * graph captured before the change), which then waits for the device once
+ * to update the table. */
 int oslmi_renderer_name_transform(oslmi_shadingsys* ss, const char* name, const float* m,
                                   const float* inv);
 /* Renderer attributes behind getattribute(name, value)

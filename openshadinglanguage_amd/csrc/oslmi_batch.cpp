@@ -312,6 +312,25 @@ struct Batch {
         return OSLMI_OK;
     }
 
+    // Named transforms and renderer attributes changed since the last
+    // upload reach the table the kernels read (the table's address never
+    // changes, so captured graphs see the new values too). Skipped while
+    // the stream is being captured: the upload waits for the device.
+    int sync_renderer_table(hipStream_t stream)
+    {
+        bool any_xf = false;
+        for (const Group* g : groups)
+            any_xf = any_xf || g->uses_xforms;
+        if (!any_xf)
+            return OSLMI_OK;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (int rc = check_hip(hipStreamIsCapturing(stream, &cs), "hipStreamIsCapturing"))
+            return rc;
+        if (cs != hipStreamCaptureStatusNone)
+            return OSLMI_OK;
+        return xform_table_sync(*ss);
+    }
+
     int launch_per_group(hipStream_t stream)
     {
         for (int k = 0; k < (int)hargs.size(); ++k)
@@ -641,6 +660,8 @@ int oslmi_batch_launch(oslmi_batch* h, void* stream)
     if (!h || !h->b.bound)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch: batch not bound");
     Batch& b = h->b;
+    if (int rc = b.sync_renderer_table((hipStream_t)stream))
+        return rc;
     if (b.gexec && b.graph_stream == (hipStream_t)stream)
         return check_hip(hipGraphLaunch(b.gexec, (hipStream_t)stream), "hipGraphLaunch");
     return b.launch_all((hipStream_t)stream);
@@ -650,6 +671,8 @@ int oslmi_batch_launch_per_group(oslmi_batch* h, void* stream)
 {
     if (!h || !h->b.bound)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch_per_group: batch not bound");
+    if (int rc = h->b.sync_renderer_table((hipStream_t)stream))
+        return rc;
     return h->b.launch_per_group((hipStream_t)stream);
 }
 
@@ -659,6 +682,8 @@ int oslmi_batch_capture(oslmi_batch* h, void* stream)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch_capture: batch not bound");
     if (!stream)
         return set_error(OSLMI_ERR_INVALID, "oslmi_batch_capture: capture needs a created stream, not the null stream");
+    if (int rc = h->b.sync_renderer_table((hipStream_t)stream))
+        return rc;
     return h->b.capture((hipStream_t)stream);
 }
 

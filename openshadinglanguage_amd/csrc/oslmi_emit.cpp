@@ -245,34 +245,94 @@ struct Emitter {
         return d.count(o) > 0;
     }
 
-    // Lobe capacity of the group's closure values: the number of closure
-    // constructors (each makes one lobe and ADD only concatenates), or the
-    // full 8 when a closure is written inside a loop.
+    // Lobe capacity of the group's closure values, bounded through the
+    // dataflow: a closure constructor makes one lobe, add holds the sum of
+    // its operands' lobes, mul and assign those of their closure operand.
+    // The ops are walked in program order (both arms of an if from the
+    // same state, merged by maximum; a loop's body repeated until nothing
+    // grows), every count only ever rising, so the result bounds each value
+    // at every point; connections carry the source's count and Ci, one
+    // variable shared by the layers, the count the layers before left.
+    // Capped at 8, the most a CompositeBSDF holds (a closure that grows in
+    // a loop reaches the cap).
     int closure_capacity() const
     {
-        int count = 0;
-        bool in_loop_write = false;
-        for (size_t L = 0; L < g.layers.size(); ++L) {
-            const std::vector<OOp>& ops = M((int)L).ops;
-            std::vector<char> inloop(ops.size(), 0);
-            for (size_t i = 0; i < ops.size(); ++i)
-                if ((ops[i].name == "for" || ops[i].name == "while" || ops[i].name == "dowhile")
-                    && ops[i].jump[3] > (int)i)
-                    for (int j = (int)i + 1; j < ops[i].jump[3] && j < (int)ops.size(); ++j)
-                        inloop[j] = 1;
-            for (size_t i = 0; i < ops.size(); ++i) {
-                const OOp& op = ops[i];
-                if (op.name == "closure")
-                    ++count;
-                if (inloop[i] && !op.args.empty() && op.args[0] >= 0
-                    && op.args[0] < (int)M((int)L).syms.size()
-                    && S((int)L, op.args[0]).type.base == kClosure)
-                    in_loop_write = true;
-            }
+        std::vector<std::vector<int>> cap(g.layers.size());
+        int ci_cap = 0;
+        for (size_t Lz = 0; Lz < g.layers.size(); ++Lz) {
+            const int L = (int)Lz;
+            cap[L].assign(M(L).syms.size(), 0);
+            for (const Connection& c : g.conns)
+                if (c.dst_layer == L && is_closure(c.src_layer, c.src_sym) && is_closure(L, c.dst_sym))
+                    cap[L][c.dst_sym] = std::max(cap[L][c.dst_sym], cap[c.src_layer][c.src_sym]);
+            const int ci = shared_ci(L);
+            if (ci >= 0)
+                cap[L][ci] = ci_cap;
+            cap_walk(L, 0, (int)M(L).ops.size(), cap[L]);
+            if (ci >= 0)
+                ci_cap = cap[L][ci];
         }
-        if (in_loop_write || count > 8)
-            return 8;
-        return count < 1 ? 1 : count;
+        int count = 1;
+        for (size_t L = 0; L < g.layers.size(); ++L)
+            for (int v : cap[L])
+                count = std::max(count, v);
+        return count;
+    }
+
+    bool is_closure(int L, int s) const
+    {
+        return s >= 0 && s < (int)M(L).syms.size() && S(L, s).type.base == kClosure;
+    }
+
+    void cap_walk(int L, int b, int e, std::vector<int>& cap) const
+    {
+        const int cap_max            = 8;
+        const std::vector<OOp>& ops = M(L).ops;
+        for (int i = b; i < e && i < (int)ops.size();) {
+            const OOp& op = ops[i];
+            if (op.name == "if" && op.jump[1] > i) {
+                std::vector<int> other = cap;
+                cap_walk(L, i + 1, op.jump[0], cap);
+                cap_walk(L, op.jump[0], op.jump[1], other);
+                for (size_t k = 0; k < cap.size(); ++k)
+                    cap[k] = std::max(cap[k], other[k]);
+                i = op.jump[1];
+                continue;
+            }
+            if ((op.name == "for" || op.name == "while" || op.name == "dowhile") && op.jump[3] > i) {
+                cap_walk(L, i + 1, op.jump[0], cap);
+                for (int round = 0; round <= cap_max + 1; ++round) {
+                    const std::vector<int> before = cap;
+                    cap_walk(L, op.jump[0], op.jump[3], cap);
+                    if (cap == before)
+                        break;
+                }
+                i = op.jump[3];
+                continue;
+            }
+            if (op.name == "functioncall" && op.jump[0] > i) {
+                cap_walk(L, i + 1, op.jump[0], cap);
+                i = op.jump[0];
+                continue;
+            }
+            ++i;
+            if (op.args.empty() || !is_closure(L, op.args[0]))
+                continue;
+            int v = op.name == "closure" ? 1 : 0;
+            if (op.name != "closure")
+                for (size_t k = 1; k < op.args.size(); ++k)
+                    if (is_closure(L, op.args[k]))
+                        v += cap[op.args[k]];
+            int& r = cap[op.args[0]];
+            r      = std::max(r, std::min(v, cap_max));
+        }
+    }
+
+    // the layer's Ci global (a closure), or -1
+    int shared_ci(int L) const
+    {
+        const int k = M(L).find("Ci");
+        return k >= 0 && S(L, k).kind == kGlobal && S(L, k).type.base == kClosure ? k : -1;
     }
 
     void analyze()
@@ -372,7 +432,7 @@ struct Emitter {
         else if (y.name == "dPdv") base = tri_global(kG_dPdv);
         else if (y.name == "Ps") base = tri_global(kG_Ps);
         else if (y.name == "time") base = f_global(kG_time);
-        else if (y.name == "Ci") return "Closure()";
+        else if (y.name == "Ci") return y.type.base == kClosure ? "ci_" : "Closure()";
         else {
             if (y.type.base == kTriple) base = "V3(0.0f)";
             else if (y.type.base == kFloat) base = "0.0f";
@@ -980,7 +1040,11 @@ struct Emitter {
                 from = "common";
             if (to == ss.commonspace)
                 to = "common";
-            if (from == to) {
+            // getmatrix multiplies the from-matrix by the to-inverse even
+            // for one space named twice (wide_opmatrix.cpp:869-885), which
+            // fails for an unknown space; only common-to-common is the
+            // identity without a table slot
+            if (from == to && from == "common") {
                 line(var(L, a(3)) + " = m44_identity();");
                 line(R + " = 1;");
                 return;
@@ -1047,7 +1111,8 @@ struct Emitter {
     }
 
     // structured emission of ops [b, e) of layer L
-    void block(int L, int b, int e, int loop_step_label, int loop_end_label, int layer_end_label)
+    void block(int L, int b, int e, int loop_step_label, int loop_end_label, int layer_end_label,
+               int return_label)
     {
         const ShaderMaster& m = M(L);
         for (int i = b; i < e && !failed;) {
@@ -1055,9 +1120,9 @@ struct Emitter {
             if (op.name == "if") {
                 int j0 = op.jump[0], j1 = op.jump[1];
                 line("if (" + ex(L, op.args[0], K_INT) + ") {");
-                block(L, i + 1, j0, loop_step_label, loop_end_label, layer_end_label);
+                block(L, i + 1, j0, loop_step_label, loop_end_label, layer_end_label, return_label);
                 line("} else {");
-                block(L, j0, j1, loop_step_label, loop_end_label, layer_end_label);
+                block(L, j0, j1, loop_step_label, loop_end_label, layer_end_label, return_label);
                 line("}");
                 i = j1;
                 continue;
@@ -1067,22 +1132,22 @@ struct Emitter {
                 int ls = ++label_counter, le = ++label_counter;
                 std::string cond = ex(L, op.args[0], K_INT);
                 line("{");
-                block(L, i + 1, j0, loop_step_label, loop_end_label, layer_end_label);  // init
+                block(L, i + 1, j0, loop_step_label, loop_end_label, layer_end_label, return_label);  // init
                 if (op.name == "dowhile") {
                     line("for (;;) {");
-                    block(L, j1, j2, ls, le, layer_end_label);  // body
+                    block(L, j1, j2, ls, le, layer_end_label, return_label);  // body
                     line(lab(ls) + ":;");
-                    block(L, j2, j3, ls, le, layer_end_label);  // step
-                    block(L, j0, j1, ls, le, layer_end_label);  // cond
+                    block(L, j2, j3, ls, le, layer_end_label, return_label);  // step
+                    block(L, j0, j1, ls, le, layer_end_label, return_label);  // cond
                     line("if (!(" + cond + ")) break;");
                     line("}");
                 } else {
                     line("for (;;) {");
-                    block(L, j0, j1, ls, le, layer_end_label);  // cond
+                    block(L, j0, j1, ls, le, layer_end_label, return_label);  // cond
                     line("if (!(" + cond + ")) break;");
-                    block(L, j1, j2, ls, le, layer_end_label);  // body
+                    block(L, j1, j2, ls, le, layer_end_label, return_label);  // body
                     line(lab(ls) + ":;");
-                    block(L, j2, j3, ls, le, layer_end_label);  // step
+                    block(L, j2, j3, ls, le, layer_end_label, return_label);  // step
                     line("}");
                 }
                 line(lab(le) + ":;");
@@ -1100,14 +1165,25 @@ struct Emitter {
                 ++i;
                 continue;
             }
-            if (op.name == "exit" || op.name == "return") {
+            if (op.name == "exit") {
                 line("goto " + lab(layer_end_label) + ";");
                 ++i;
                 continue;
             }
+            if (op.name == "return") {
+                // the end of the innermost function call, or of the layer
+                line("goto " + lab(return_label) + ";");
+                ++i;
+                continue;
+            }
             if (op.name == "functioncall") {
-                // inline the function body [i+1, jump0)
-                block(L, i + 1, op.jump[0], loop_step_label, loop_end_label, layer_end_label);
+                // inline the function body [i+1, jump0); a return inside it
+                // resumes after the call (llvm_gen.cpp:4327-4360, :4396-4411)
+                const int fe = ++label_counter;
+                line("{");
+                block(L, i + 1, op.jump[0], loop_step_label, loop_end_label, layer_end_label, fe);
+                line(lab(fe) + ":;");
+                line("}");
                 i = op.jump[0];
                 continue;
             }
@@ -1175,9 +1251,13 @@ struct Emitter {
             return false;
         }
         g.output_planes = plane;
-        int last        = (int)g.layers.size() - 1;
-        int ci          = last >= 0 ? M(last).find("Ci") : -1;
-        g.surface_closure = ci >= 0 && S(last, ci).type.base == kClosure;
+        // Ci is one variable of the group: every layer starts from the value
+        // the layers before it left and hands its own on (a global, shared
+        // by all layers as in the reference's ShaderGlobals)
+        g.surface_closure = false;
+        for (int L = 0; L < (int)g.layers.size(); ++L)
+            if (shared_ci(L) >= 0)
+                g.surface_closure = true;
 
         // body
         std::ostringstream body;
@@ -1187,7 +1267,7 @@ struct Emitter {
             line("{");
             declare_layer(L);
             int le = ++label_counter;
-            block(L, M(L).main_begin, (int)M(L).ops.size(), -1, -1, le);
+            block(L, M(L).main_begin, (int)M(L).ops.size(), -1, -1, le, le);
             line(lab(le) + ":;");
             // hand layer outputs to downstream layers
             for (const Connection& c : g.conns)
@@ -1205,8 +1285,8 @@ struct Emitter {
                         line("A.out[" + std::to_string(o.first_plane) + "][i] = (float)(" + v + ");");
                     }
                 }
-            if (L == last && g.surface_closure)
-                line("ci_ = " + var(L, ci) + ";");
+            if (shared_ci(L) >= 0)
+                line("ci_ = " + var(L, shared_ci(L)) + ";");
             line("}");
         }
         std::swap(os, body);

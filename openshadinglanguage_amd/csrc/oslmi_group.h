@@ -58,7 +58,7 @@ struct Group {
     std::string source;
     std::vector<OutputSlot> outputs;
     int output_planes        = 0;
-    bool surface_closure     = false;  // last layer writes Ci
+    bool surface_closure     = false;  // some layer has the Ci global (shared by all layers)
     bool uses_colorsys       = false;
     bool uses_xforms         = false;  // reads the transform / attribute table
     bool can_sample          = false;  // kernel samples Ci when A.samp is set

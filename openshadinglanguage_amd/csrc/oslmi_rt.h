@@ -72,7 +72,9 @@ OSLMI_DEV Dv neg(const Dv& a) { return -a; }
 OSLMI_DEV int neg(int a) { return -a; }
 
 // OSL mod: a - b*floor(a/b) (safe: 0 when b == 0)
-OSLMI_DEV float fmod_osl(float a, float b) { return b != 0.0f ? a - b * floorf(a / b) : 0.0f; }
+// the mod op on floats: the exact remainder of a / b with the sign of a
+// (LLVM frem, batched_llvm_gen.cpp:1683-1750), 0 when b is 0
+OSLMI_DEV float fmod_osl(float a, float b) { return b != 0.0f ? fmodf(a, b) : 0.0f; }
 
 // ---------------------------------------------------------- comparisons
 template<class A, class B> OSLMI_DEV int lt(A a, B b) { return val(a) < val(b); }

@@ -458,11 +458,6 @@ struct Exec {
         if (o == "assign") {
             if (rb == kClosure) {
                 vals[L][r].cl = vals[L][a(1)].cl;
-                // Ci of the surface layer
-                if (G.layers[L].m->syms[r].name == "Ci" && G.layers[L].m->syms[r].kind == "global") {
-                    has_ci = true;
-                    ci     = vals[L][r].cl;
-                }
             } else if (rb == kTriple) {
                 setT(L, r, T(L, a(1)));
             } else if (rb == kInt) {
@@ -509,6 +504,21 @@ struct Exec {
             }
             const float x = F(L, a(1)), y = F(L, a(2));
             setF(L, r, o == "add" ? x + y : o == "sub" ? x - y : o == "mul" ? x * y : x / y);
+            return;
+        }
+        if (o == "mod") {
+            // int: a % b; float: the exact remainder with a's sign (LLVM
+            // frem); 0 when b is 0 (batched_llvm_gen.cpp:1683-1750)
+            auto fm = [](float x, float y) { return y != 0.0f ? std::fmod(x, y) : 0.0f; };
+            if (rb == kInt) {
+                const int x = I(L, a(1)), y = I(L, a(2));
+                vals[L][r].i = y != 0 ? x % y : 0;
+            } else if (rb == kTriple) {
+                const V3 x = T(L, a(1)), y = T(L, a(2));
+                setT(L, r, V3(fm(x.x, y.x), fm(x.y, y.y), fm(x.z, y.z)));
+            } else {
+                setF(L, r, fm(F(L, a(1)), F(L, a(2))));
+            }
             return;
         }
         if (o == "neg") {
@@ -661,7 +671,9 @@ struct Exec {
         if (o == "getmatrix") {
             const std::string from = canon(Sv(L, a(1))), to = canon(Sv(L, a(2)));
             Mat4 M   = mat4_identity();
-            bool ok  = from == to || from_to(from, to, M);
+            // Mfrom * Mto^-1 for any pair, one space named twice included
+            // (wide_opmatrix.cpp:869-885)
+            bool ok  = from_to(from, to, M);
             for (int i = 0; i < 4; ++i)
                 for (int j = 0; j < 4; ++j)
                     vals[L][a(3)].f[4 * i + j] = M.x[i][j];
@@ -712,53 +724,101 @@ struct Exec {
         abort();
     }
 
-    // executes ops [b, e) of layer L with structured control flow
-    void block(int L, int b, int e)
+    // How a block ended: ran to its end, or left by break / continue (to
+    // the innermost loop), return (to the end of the innermost
+    // functioncall, or of the layer at its top level) or exit (the layer).
+    enum Flow { kNext, kBreak, kContinue, kReturn, kExit };
+
+    // executes ops [b, e) of layer L with structured control flow, as the
+    // reference lowers it (llvm_gen.cpp: loops :2495-2535, break/continue
+    // :2539-2551, functioncall :4327-4360, return/exit :4396-4411)
+    Flow block(int L, int b, int e)
     {
         const Master& m = *G.layers[L].m;
         for (int i = b; i < e;) {
             const Op& op = m.ops[i];
             if (op.name == "if") {
-                if (I(L, op.args[0]))
-                    block(L, i + 1, op.jump[0]);
-                else
-                    block(L, op.jump[0], op.jump[1]);
+                const Flow f = I(L, op.args[0]) ? block(L, i + 1, op.jump[0])
+                                                : block(L, op.jump[0], op.jump[1]);
+                if (f != kNext)
+                    return f;
                 i = op.jump[1];
                 continue;
             }
-            if (op.name == "for" || op.name == "while") {
-                block(L, i + 1, op.jump[0]);
+            if (op.name == "for" || op.name == "while" || op.name == "dowhile") {
+                // [init i+1, j0) [cond j0, j1) [body j1, j2) [step j2, j3)
+                Flow f = block(L, i + 1, op.jump[0]);
+                if (f != kNext)
+                    return f;
+                bool first = op.name == "dowhile";  // do-while enters at the body
                 for (int guard = 0; guard < 1000000; ++guard) {
-                    block(L, op.jump[0], op.jump[1]);
-                    if (!I(L, op.args[0]))
+                    if (!first) {
+                        f = block(L, op.jump[0], op.jump[1]);
+                        if (f != kNext)
+                            return f;
+                        if (!I(L, op.args[0]))
+                            break;
+                    }
+                    first = false;
+                    f     = block(L, op.jump[1], op.jump[2]);
+                    if (f == kBreak)
                         break;
-                    block(L, op.jump[1], op.jump[2]);
-                    block(L, op.jump[2], op.jump[3]);
+                    if (f == kReturn || f == kExit)
+                        return f;
+                    f = block(L, op.jump[2], op.jump[3]);  // step (also after continue)
+                    if (f != kNext)
+                        return f;
                 }
                 i = op.jump[3];
+                continue;
+            }
+            if (op.name == "break")
+                return kBreak;
+            if (op.name == "continue")
+                return kContinue;
+            if (op.name == "return")
+                return kReturn;
+            if (op.name == "exit")
+                return kExit;
+            if (op.name == "functioncall") {
+                // the function body [i+1, j0) inline; its return ends it
+                const Flow f = block(L, i + 1, op.jump[0]);
+                if (f == kExit)
+                    return f;
+                i = op.jump[0];
                 continue;
             }
             op1(L, op);
             ++i;
         }
+        return kNext;
     }
 
     void run()
     {
         vals.assign(G.layers.size(), {});
         has_ci = false;
-        for (size_t L = 0; L < G.layers.size(); ++L) {
-            init_layer((int)L);
-            block((int)L, 0, (int)G.layers[L].m->ops.size());
-            // Ci written by any closure op (add, mul, closure), not only
-            // by an assign: its final value is the layer's closure
+        ci     = Composite();
+        // Ci is one global of the shading point, shared by the layers: each
+        // layer starts from the value the layers before it left
+        auto ci_sym = [&](int L) {
             const Master& m = *G.layers[L].m;
             for (size_t k = 0; k < m.syms.size(); ++k)
                 if (m.syms[k].name == "Ci" && m.syms[k].kind == "global"
-                    && m.syms[k].base == kClosure && vals[L][k].cl.n > 0) {
-                    has_ci = true;
-                    ci     = vals[L][k].cl;
-                }
+                    && m.syms[k].base == kClosure)
+                    return (int)k;
+            return -1;
+        };
+        for (size_t L = 0; L < G.layers.size(); ++L) {
+            init_layer((int)L);
+            const int k = ci_sym((int)L);
+            if (k >= 0)
+                vals[L][k].cl = ci;
+            block((int)L, 0, (int)G.layers[L].m->ops.size());
+            if (k >= 0) {
+                has_ci = true;
+                ci     = vals[L][k].cl;
+            }
         }
     }
 };

@@ -185,3 +185,22 @@ def test_beckmann_emits_and_rejects_sampling():
     g2 = S.build_generated_group(ss2, ([(o.name, "l0", o.text())], []), "bk_s")
     with pytest.raises(OSLMIError):
         ss2.emit(g2)
+
+
+def test_closure_capacity_follows_dataflow():
+    """The lobe capacity the emitter gives closure values counts lobes
+    through add/mul/assign and connections: a closure reused in an add
+    needs two slots, one grown in a loop the composite's maximum of 8, and
+    Ci written by an upstream layer is the group's surface closure."""
+    import _cfprogs as C
+    for prog, cap in ((C.closure_reuse, 2), (C.closure_loop, 8)):
+        o, _ = prog()
+        ss = S.ShadingSystem()
+        g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), o.name)
+        assert "using Closure = LobeList<%d>;" % cap in ss.emit(g)
+    osos, conns, _ = C.closure_upstream_layers()
+    ss = S.ShadingSystem()
+    layers = [(o.name, "l%d" % k, o.text()) for k, o in enumerate(osos)]
+    src = ss.emit(S.build_generated_group(ss, (layers, conns), "up"))
+    assert "using Closure = LobeList<2>;" in src
+    assert src.count("ci_ = ") == 2  # both layers hand Ci on
