@@ -87,4 +87,204 @@ OSLMI_DEV V3 blackbody_rgb(const float* table, const float* xyz2rgb, const float
     return V3(rgb.x < 0.0f ? 0.0f : rgb.x, rgb.y < 0.0f ? 0.0f : rgb.y, rgb.z < 0.0f ? 0.0f : rgb.z);
 }
 
+// ---- color-space transforms (opcolor_impl.h:246-490) -------------------
+// Each transform is written once over a scalar type F, float for values or
+// Df for Dual2 colors, so both follow the reference's own evaluation order
+// (with derivatives carried by Dual2 arithmetic, dual.h). Spaces: rgb (the
+// working space, also "RGB", "linear" and the system's own name), hsv,
+// hsl, YIQ, XYZ, xyY, sRGB.
+enum ColorSpaceCode { kCS_rgb = 0, kCS_hsv, kCS_hsl, kCS_YIQ, kCS_XYZ, kCS_xyY, kCS_sRGB };
+
+template<class F> struct CTrip {
+    F x, y, z;
+};
+OSLMI_DEV float cs_val(float x) { return x; }
+OSLMI_DEV float cs_val(const Df& x) { return x.v; }
+OSLMI_DEV CTrip<float> cs_split(V3 c) { return { c.x, c.y, c.z }; }
+OSLMI_DEV CTrip<Df> cs_split(const Dv& c) { return { comp(c, 0), comp(c, 1), comp(c, 2) }; }
+OSLMI_DEV V3 cs_join(const CTrip<float>& c) { return V3(c.x, c.y, c.z); }
+OSLMI_DEV Dv cs_join(const CTrip<Df>& c) { return make_dv(c.x, c.y, c.z); }
+// sfm::min_val / max_val (sfmath.h:219-229): the right operand unless the
+// comparison picks the left
+template<class F> OSLMI_DEV F cs_min(const F& l, const F& r) { return cs_val(r) > cs_val(l) ? l : r; }
+template<class F> OSLMI_DEV F cs_max(const F& l, const F& r) { return cs_val(r) > cs_val(l) ? r : l; }
+
+// OpenImageIO's safe_pow: 1 for y == 0, 0 for x == 0 or a negative x with
+// a non-integer y, otherwise pow clamped to the finite range
+OSLMI_DEV float cs_safe_pow(float x, float y)
+{
+    if (y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f)
+        return 0.0f;
+    if (x < 0.0f && y != floorf(y))
+        return 0.0f;
+    float r = powf(x, y);
+    const float big = 3.40282346638528859812e+38f;
+    return r < -big ? -big : (r > big ? big : r);
+}
+// Dual2 safe_pow with a constant exponent (dual.h:1059-1067): the value is
+// formed as u^(v-1) * u
+OSLMI_DEV float cs_pow(float u, float v) { return cs_safe_pow(u, v); }
+OSLMI_DEV Df cs_pow(const Df& u, float v)
+{
+    float pm1  = cs_safe_pow(u.v, v - 1.0f);
+    float puv  = pm1 * u.v;
+    float logu = u.v > 0.0f ? logf(u.v < 1.17549435e-38f ? 1.17549435e-38f : u.v) : 0.0f;
+    float dfdu = v * pm1, dfdv = logu * puv;
+    return Df(puv, dfdu * u.dx + dfdv * 0.0f, dfdu * u.dy + dfdv * 0.0f);
+}
+OSLMI_DEV float cs_floor(float x) { return floorf(x); }
+OSLMI_DEV float cs_floor(const Df& x) { return floorf(x.v); }  // loses derivatives
+
+// row 3-vector times a 3x3 matrix given row-major (Imath's v * M)
+template<class F>
+OSLMI_DEV CTrip<F> cs_mul(const CTrip<F>& c, const float m[9])
+{
+    return { c.x * m[0] + c.y * m[3] + c.z * m[6], c.x * m[1] + c.y * m[4] + c.z * m[7],
+             c.x * m[2] + c.y * m[5] + c.z * m[8] };
+}
+
+template<class F> OSLMI_DEV CTrip<F> hsv_to_rgb(const CTrip<F>& hsv)
+{
+    F h = hsv.x, s = hsv.y, v = hsv.z;
+    if (cs_val(s) < 0.0001f)
+        return { v, v, v };
+    h      = 6.0f * (h - cs_floor(h));
+    int hi = (int)floorf(cs_val(h));
+    F f    = h - F((float)hi);
+    F p    = v * (1.0f - s);
+    F q    = v * (1.0f - s * f);
+    F t    = v * (1.0f - s * (1.0f - f));
+    if (hi == 0)
+        return { v, t, p };
+    if (hi == 1)
+        return { q, v, p };
+    if (hi == 2)
+        return { p, v, t };
+    if (hi == 3)
+        return { p, q, v };
+    if (hi == 4)
+        return { t, p, v };
+    return { v, p, q };
+}
+
+template<class F> OSLMI_DEV CTrip<F> rgb_to_hsv(const CTrip<F>& rgb)
+{
+    F r = rgb.x, g = rgb.y, b = rgb.z;
+    F mincomp = cs_min(r, cs_min(g, b));
+    F maxcomp = cs_max(r, cs_max(g, b));
+    F delta   = maxcomp - mincomp;
+    F v       = maxcomp;
+    F s       = F(0.0f);
+    if (cs_val(maxcomp) > 0.0f)
+        s = delta / maxcomp;
+    F h = F(0.0f);
+    if (cs_val(s) > 0.0f) {
+        float k;
+        F x, y;
+        if (cs_val(r) >= cs_val(maxcomp)) {
+            k = 0.0f / 6.0f, x = g, y = b;
+        } else if (cs_val(g) >= cs_val(maxcomp)) {
+            k = 2.0f / 6.0f, x = b, y = r;
+        } else {
+            k = 4.0f / 6.0f, x = r, y = g;
+        }
+        h = k + (x - y) / (6.0f * delta);
+        if (cs_val(h) < 0.0f)
+            h = h + 1.0f;
+    }
+    return { h, s, v };
+}
+
+template<class F> OSLMI_DEV CTrip<F> hsl_to_rgb(const CTrip<F>& hsl)
+{
+    F h = hsl.x, s = hsl.y, l = hsl.z;
+    F v = (cs_val(l) <= 0.5f) ? (l * (1.0f + s)) : (l * (1.0f - s) + s);
+    if (cs_val(v) <= 0.0f)
+        return { F(0.0f), F(0.0f), F(0.0f) };
+    F mn = 2.0f * l - v;
+    s    = (v - mn) / v;
+    return hsv_to_rgb(CTrip<F>{ h, s, v });
+}
+
+template<class F> OSLMI_DEV CTrip<F> rgb_to_hsl(const CTrip<F>& rgb)
+{
+    F minval      = cs_min(rgb.x, cs_min(rgb.y, rgb.z));
+    CTrip<F> hsv  = rgb_to_hsv(rgb);
+    F maxval      = hsv.z;
+    F h = hsv.x, s, l = (minval + maxval) / 2.0f;
+    if (cs_val(minval) == cs_val(maxval)) {
+        s = F(0.0f);
+    } else {
+        F min2max = maxval - minval;
+        F divisor = cs_val(l) <= 0.5f ? maxval + minval : 2.0f - maxval - minval;
+        s         = min2max / divisor;
+    }
+    return { h, s, l };
+}
+
+// YIQ matrices (opcolor_impl.h:408-424), row-major
+template<class F> OSLMI_DEV CTrip<F> yiq_to_rgb(const CTrip<F>& c)
+{
+    const float m[9] = { 1.0000f, 1.0000f, 1.0000f, 0.9557f, -0.2716f, -1.1082f, 0.6199f, -0.6469f, 1.7051f };
+    return cs_mul(c, m);
+}
+template<class F> OSLMI_DEV CTrip<F> rgb_to_yiq(const CTrip<F>& c)
+{
+    const float m[9] = { 0.299f, 0.596f, 0.212f, 0.587f, -0.275f, -0.523f, 0.114f, -0.321f, 0.311f };
+    return cs_mul(c, m);
+}
+
+template<class F> OSLMI_DEV CTrip<F> xyz_to_xyY(const CTrip<F>& c)
+{
+    F n     = c.x + c.y + c.z;
+    F n_inv = cs_val(n) >= 1.0e-6f ? 1.0f / n : F(0.0f);
+    return { c.x * n_inv, c.y * n_inv, c.y };
+}
+template<class F> OSLMI_DEV CTrip<F> xyY_to_xyz(const CTrip<F>& c)
+{
+    F Y   = c.z;
+    F Y_y = cs_val(c.y) > 1.0e-6f ? Y / c.y : F(0.0f);
+    F X   = Y_y * c.x;
+    F Z   = Y_y * (1.0f - c.x - c.y);
+    return { X, Y, Z };
+}
+template<class F> OSLMI_DEV F srgb_to_linear1(const F& x)
+{
+    return cs_val(x) <= 0.04045f ? x * (1.0f / 12.92f)
+                                 : cs_pow((x + 0.055f) * (1.0f / 1.055f), 2.4f);
+}
+template<class F> OSLMI_DEV F linear_to_srgb1(const F& x)
+{
+    return cs_val(x) <= 0.0031308f ? 12.92f * x : 1.055f * cs_pow(x, 1.f / 2.4f) - 0.055f;
+}
+
+// ColorSystem::transformc (opcolor.cpp:324-380): from -> rgb -> to
+template<class F>
+OSLMI_DEV CTrip<F> cs_transform(const ColorSysDev* CS, int from, int to, CTrip<F> c)
+{
+    if (from == kCS_hsv) c = hsv_to_rgb(c);
+    else if (from == kCS_hsl) c = hsl_to_rgb(c);
+    else if (from == kCS_YIQ) c = yiq_to_rgb(c);
+    else if (from == kCS_XYZ) c = cs_mul(c, CS->xyz2rgb);
+    else if (from == kCS_xyY) c = cs_mul(xyY_to_xyz(c), CS->xyz2rgb);
+    else if (from == kCS_sRGB) c = { srgb_to_linear1(c.x), srgb_to_linear1(c.y), srgb_to_linear1(c.z) };
+    if (to == kCS_hsv) c = rgb_to_hsv(c);
+    else if (to == kCS_hsl) c = rgb_to_hsl(c);
+    else if (to == kCS_YIQ) c = rgb_to_yiq(c);
+    else if (to == kCS_XYZ) c = cs_mul(c, CS->rgb2xyz);
+    else if (to == kCS_xyY) c = xyz_to_xyY(cs_mul(c, CS->rgb2xyz));
+    else if (to == kCS_sRGB) c = { linear_to_srgb1(c.x), linear_to_srgb1(c.y), linear_to_srgb1(c.z) };
+    return c;
+}
+template<int FROM, int TO> OSLMI_DEV V3 transformc(const ColorSysDev* CS, V3 c)
+{
+    return cs_join(cs_transform<float>(CS, FROM, TO, cs_split(c)));
+}
+template<int FROM, int TO> OSLMI_DEV Dv transformc(const ColorSysDev* CS, const Dv& c)
+{
+    return cs_join(cs_transform<Df>(CS, FROM, TO, cs_split(c)));
+}
+
 }  // namespace oslmi

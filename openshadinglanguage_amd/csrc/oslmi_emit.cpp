@@ -328,6 +328,22 @@ struct Emitter {
         }
     }
 
+    // code of a color space name (ColorSpaceCode of the device library), or
+    // -1 for one only OpenColorIO knows. transformc also takes "linear" and
+    // "sRGB"; the color constructor's to_rgb neither (opcolor.cpp:275-296).
+    int color_space_code(const std::string& n, bool transformc) const
+    {
+        if (n == "rgb" || n == "RGB" || n == ss.colorspace || (transformc && n == "linear"))
+            return 0;
+        if (n == "hsv") return 1;
+        if (n == "hsl") return 2;
+        if (n == "YIQ") return 3;
+        if (n == "XYZ") return 4;
+        if (n == "xyY") return 5;
+        if (n == "sRGB" && transformc) return 6;
+        return -1;
+    }
+
     // the layer's Ci global (a closure), or -1
     int shared_ci(int L) const
     {
@@ -597,8 +613,8 @@ struct Emitter {
         }
         if (nm == "simplex" || nm == "usimplex") {
             bool uns = nm == "usimplex";
-            if (dim != 3 && !(dim == 1 || dim == 2)) {
-                fail("simplex noise supports 1-3 dimensions");
+            if (dim < 1 || dim > 4) {
+                fail("simplex noise supports 1-4 dimensions");
                 return;
             }
             line("{");
@@ -607,6 +623,8 @@ struct Emitter {
                 if (rd) {
                     if (dim == 3) {
                         line("  Df s" + ks + "_ = simplex3_d(" + join(3) + ", " + ks + "u);");
+                    } else if (dim == 4) {
+                        line("  Df s" + ks + "_ = simplex4_d(" + join(4) + ", " + ks + "u);");
                     } else {
                         line("  float d" + ks + "_[3] = {0.f, 0.f, 0.f};");
                         if (dim == 1)
@@ -625,7 +643,8 @@ struct Emitter {
                 } else {
                     std::string call = dim == 1   ? "simplex1(" + join(1) + ", " + ks + "u, nullptr)"
                                        : dim == 2 ? "simplex2(" + join(2) + ", " + ks + "u, nullptr)"
-                                                  : "simplex3<false>(" + join(3) + ", " + ks + "u, nullptr)";
+                                       : dim == 3 ? "simplex3<false>(" + join(3) + ", " + ks + "u, nullptr)"
+                                                  : "simplex4<false>(" + join(4) + ", " + ks + "u, nullptr)";
                     line("  float s" + ks + "_ = " + call + ";");
                     if (uns)
                         line("  s" + ks + "_ = 0.5f * (s" + ks + "_ + 1.0f);");
@@ -817,6 +836,27 @@ struct Emitter {
             line("compassign(" + R + ", " + ex(L, a(1), K_INT) + ", " + ex(L, a(2), ck) + ");");
             return;
         }
+        if (o == "color" && op.args.size() == 5 && S(L, a(1)).type.base == kString) {
+            // color(space, a, b, c): the components converted to rgb by
+            // ColorSystem::to_rgb (opcolor.cpp:275-296); derivatives of the
+            // result are zero (batched_llvm_gen.cpp:2740-2790)
+            std::string space;
+            if (!strval(L, a(1), space)) {
+                fail("color constructor needs a constant space name");
+                return;
+            }
+            const int fc = color_space_code(space, false);
+            if (fc < 0) {
+                fail("color space '" + space + "' needs OpenColorIO");
+                return;
+            }
+            uses_cs = true;
+            const std::string t = "V3(" + ex(L, a(2), K_FLOAT) + ", " + ex(L, a(3), K_FLOAT) + ", "
+                                  + ex(L, a(4), K_FLOAT) + ")";
+            const std::string v = fc == 0 ? t : "transformc<" + std::to_string(fc) + ", 0>(CS, " + t + ")";
+            line(R + " = " + (rk == K_DV ? "Dv(" + v + ")" : v) + ";");
+            return;
+        }
         if (o == "color" || o == "point" || o == "vector" || o == "normal") {
             if (op.args.size() == 4) {
                 Kind ck = rk == K_DV ? K_DF : K_FLOAT;
@@ -898,33 +938,32 @@ struct Emitter {
             return;
         }
         if (o == "transformc") {
+            // transformc [from] to C (ColorSystem::transformc, opcolor.cpp:
+            // 324-380): from -> rgb -> to, with derivatives when C has them
             uses_cs = true;
             std::string from = "rgb", to;
             int cidx = 2;
             if (op.args.size() == 4) {
-                strval(L, a(1), from);
-                strval(L, a(2), to);
+                if (!strval(L, a(1), from) || !strval(L, a(2), to)) {
+                    fail("transformc needs constant space names");
+                    return;
+                }
                 cidx = 3;
-            } else {
-                strval(L, a(1), to);
-            }
-            auto is_rgb = [&](const std::string& s) {
-                return s == "rgb" || s == "RGB" || s == "linear" || s == ss.colorspace;
-            };
-            std::string c = ex(L, a(cidx), rk);
-            auto mat = [&](const char* m, const std::string& x) {
-                if (rk == K_DV)
-                    return "Dv(mul_row((" + x + ").v, CS->" + m + "), mul_row((" + x + ").dx, CS->" + m + "), mul_row((" + x + ").dy, CS->" + m + "))";
-                return "mul_row(" + x + ", CS->" + m + ")";
-            };
-            std::string v = c;
-            if (from == "XYZ" && is_rgb(to)) v = mat("xyz2rgb", c);
-            else if (is_rgb(from) && to == "XYZ") v = mat("rgb2xyz", c);
-            else if (!(is_rgb(from) && is_rgb(to)) && from != to) {
-                fail("transformc " + from + " -> " + to + " is not supported");
+            } else if (!strval(L, a(1), to)) {
+                fail("transformc needs constant space names");
                 return;
             }
-            line(R + " = " + v + ";");
+            const int fc = color_space_code(from, true), tc = color_space_code(to, true);
+            if (fc < 0 || tc < 0) {
+                fail("transformc " + from + " -> " + to
+                     + ": spaces beyond rgb/hsv/hsl/YIQ/XYZ/xyY/sRGB need OpenColorIO");
+                return;
+            }
+            const std::string c = ex(L, a(cidx), rk);
+            if (fc == 0 && tc == 0)  // other pairs go through rgb, as the reference does
+                line(R + " = " + c + ";");
+            else
+                line(R + " = transformc<" + std::to_string(fc) + ", " + std::to_string(tc) + ">(CS, " + c + ");");
             return;
         }
         if (o == "transform" || o == "transformv" || o == "transformn") {

@@ -12,7 +12,7 @@
 //   CellNoise/HashNoise         src/include/OSL/oslnoise.h:324-480
 //   floorfrac/fade/grad/scale   src/include/OSL/oslnoise.h:872-1041
 //   perlin<CGScalar> 1-4D       src/include/OSL/oslnoise.h:1327-2282
-//   simplexnoise1/2/3           src/include/OSL/sfm_simplex.h:559-960
+//   simplexnoise1/2/3/4         src/include/OSL/sfm_simplex.h:559-1230
 #pragma once
 #include "oslmi_types.h"
 
@@ -542,6 +542,104 @@ OSLMI_DEV float simplex3(float x, float y, float z, uint32_t seed, float* dnoise
         dnoise[2] = dz * scale;
     }
     return noise;
+}
+
+// 4D gradients: the 32 edges of the tesseract (sfm_simplex.h:184-193) as
+// per-component masks over h & 31 (bit set in P / N: component +1 / -1).
+OSLMI_DEV void simplex_grad4(uint32_t h, float g[4])
+{
+    uint32_t bit = 1u << (h & 31u);
+    g[0] = (bit & 0x0F0F0F00u) ? 1.0f : ((bit & 0xF0F0F000u) ? -1.0f : 0.0f);
+    g[1] = (bit & 0x3333000Fu) ? 1.0f : ((bit & 0xCCCC00F0u) ? -1.0f : 0.0f);
+    g[2] = (bit & 0x55003333u) ? 1.0f : ((bit & 0xAA00CCCCu) ? -1.0f : 0.0f);
+    g[3] = (bit & 0x00555555u) ? 1.0f : ((bit & 0x00AAAAAAu) ? -1.0f : 0.0f);
+}
+
+// 4D simplex noise (simplexnoise4, sfm_simplex.h:970-1230). The simplex's
+// corner order comes from each coordinate's rank among x0, y0, z0, w0 under
+// the reference's six strict comparisons (a coordinate that loses a tie
+// ranks below); corner q (1..3) steps along the coordinates ranked 4 - q
+// and above. This equals the reference's 64-entry corner table for every
+// comparison pattern real inputs produce.
+template<bool DERIVS>
+OSLMI_DEV float simplex4(float x, float y, float z, float w, uint32_t seed, float* dnoise)
+{
+    const float F4 = 0.309016994f;
+    const float G4 = 0.138196601f;
+    float s        = (x + y + z + w) * F4;
+    float xs = x + s, ys = y + s, zs = z + s, ws = w + s;
+    int i = ifloor(xs), j = ifloor(ys), k = ifloor(zs), l = ifloor(ws);
+    float t  = (float)(i + j + k + l) * G4;
+    float x0 = x - ((float)i - t), y0 = y - ((float)j - t);
+    float z0 = z - ((float)k - t), w0 = w - ((float)l - t);
+    const int xy = x0 > y0, xz = x0 > z0, yz = y0 > z0, xw = x0 > w0, yw = y0 > w0, zw = z0 > w0;
+    const int rx = xy + xz + xw;
+    const int ry = (1 - xy) + yz + yw;
+    const int rz = (1 - xz) + (1 - yz) + zw;
+    const int rw = (1 - xw) + (1 - yw) + (1 - zw);
+    float cx[5], cy[5], cz[5], cw[5];
+    int ci[5][4];
+    cx[0] = x0, cy[0] = y0, cz[0] = z0, cw[0] = w0;
+    ci[0][0] = i, ci[0][1] = j, ci[0][2] = k, ci[0][3] = l;
+#pragma unroll
+    for (int q = 1; q <= 3; ++q) {
+        const int oi = rx >= 4 - q, oj = ry >= 4 - q, ok = rz >= 4 - q, ol = rw >= 4 - q;
+        const float gq = q == 1 ? G4 : (q == 2 ? 2.0f * G4 : 3.0f * G4);
+        cx[q] = x0 - (float)oi + gq;
+        cy[q] = y0 - (float)oj + gq;
+        cz[q] = z0 - (float)ok + gq;
+        cw[q] = w0 - (float)ol + gq;
+        ci[q][0] = i + oi, ci[q][1] = j + oj, ci[q][2] = k + ok, ci[q][3] = l + ol;
+    }
+    cx[4] = x0 - 1.0f + 4.0f * G4, cy[4] = y0 - 1.0f + 4.0f * G4;
+    cz[4] = z0 - 1.0f + 4.0f * G4, cw[4] = w0 - 1.0f + 4.0f * G4;
+    ci[4][0] = i + 1, ci[4][1] = j + 1, ci[4][2] = k + 1, ci[4][3] = l + 1;
+    float g[5][4], tt[5], t2[5], t4[5];
+    float noise_sum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const uint32_t kl = scramble((uint32_t)ci[q][2], (uint32_t)ci[q][3], seed);
+        simplex_grad4(scramble((uint32_t)ci[q][0], (uint32_t)ci[q][1], kl), g[q]);
+        tt[q] = 0.5f - cx[q] * cx[q] - cy[q] * cy[q] - cz[q] * cz[q] - cw[q] * cw[q];
+        t2[q] = tt[q] * tt[q];
+        t4[q] = t2[q] * t2[q];
+        float nq = t4[q] * (g[q][0] * cx[q] + g[q][1] * cy[q] + g[q][2] * cz[q] + g[q][3] * cw[q]);
+        if (tt[q] < 0.0f)
+            nq = 0.0f;
+        noise_sum = q == 0 ? nq : noise_sum + nq;
+    }
+    const float scale = 54.0f;
+    float noise       = scale * noise_sum;
+    if (DERIVS) {
+        float d[4];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            if (tt[q] < 0.0f)
+                g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0f;
+            float temp = t2[q] * tt[q]
+                         * (g[q][0] * cx[q] + g[q][1] * cy[q] + g[q][2] * cz[q] + g[q][3] * cw[q]);
+            d[0] = q == 0 ? temp * cx[q] : d[0] + temp * cx[q];
+            d[1] = q == 0 ? temp * cy[q] : d[1] + temp * cy[q];
+            d[2] = q == 0 ? temp * cz[q] : d[2] + temp * cz[q];
+            d[3] = q == 0 ? temp * cw[q] : d[3] + temp * cw[q];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            d[a] *= -8.0f;
+            d[a] += t4[0] * g[0][a] + t4[1] * g[1][a] + t4[2] * g[2][a] + t4[3] * g[3][a]
+                    + t4[4] * g[4][a];
+            dnoise[a] = d[a] * scale;
+        }
+    }
+    return noise;
+}
+
+OSLMI_DEV Df simplex4_d(const Df& x, const Df& y, const Df& z, const Df& w, uint32_t seed)
+{
+    float d[4];
+    float r = simplex4<true>(x.v, y.v, z.v, w.v, seed, d);
+    return Df(r, d[0] * x.dx + d[1] * y.dx + d[2] * z.dx + d[3] * w.dx,
+              d[0] * x.dy + d[1] * y.dy + d[2] * z.dy + d[3] * w.dy);
 }
 
 // Dual2 simplex via the chain rule (SimplexNoiseScalar, oslnoise.h:2925-2957)

@@ -65,7 +65,8 @@ __global__ __launch_bounds__(256) void k_noise(const float* __restrict__ in,
             for (int c = 0; c < OUTDIM; ++c) {
                 r[c] = INDIM == 1   ? simplex1(p[0], (uint32_t)c, nullptr)
                        : INDIM == 2 ? simplex2(p[0], p[1], (uint32_t)c, nullptr)
-                                    : simplex3<false>(p[0], p[1], p[2], (uint32_t)c, nullptr);
+                       : INDIM == 3 ? simplex3<false>(p[0], p[1], p[2], (uint32_t)c, nullptr)
+                                    : simplex4<false>(p[0], p[1], p[2], p[3], (uint32_t)c, nullptr);
                 if (KIND == kUSimplex)
                     r[c] = 0.5f * (r[c] + 1.0f);
             }
@@ -116,8 +117,10 @@ __global__ __launch_bounds__(256) void k_noise_d(const float* __restrict__ in,
                 } else if (INDIM == 2) {
                     v    = simplex2(p[0].v, p[1].v, (uint32_t)c, d);
                     r[c] = Df(v, d[0] * p[0].dx + d[1] * p[1].dx, d[0] * p[0].dy + d[1] * p[1].dy);
-                } else {
+                } else if (INDIM == 3) {
                     r[c] = simplex3_d(p[0], p[1], p[2], (uint32_t)c);
+                } else {
+                    r[c] = simplex4_d(p[0], p[1], p[2], p[3], (uint32_t)c);
                 }
                 if (KIND == kUSimplex)
                     r[c] = Df(0.5f * (r[c].v + 1.0f), 0.5f * r[c].dx, 0.5f * r[c].dy);
@@ -164,7 +167,7 @@ template<int KIND, int OUTDIM> static NoiseKernel pick_plain(int indim)
     case 1: return k_noise<KIND, OUTDIM, 1>;
     case 2: return k_noise<KIND, OUTDIM, 2>;
     case 3: return k_noise<KIND, OUTDIM, 3>;
-    case 4: return KIND >= kSimplex ? nullptr : k_noise<KIND, OUTDIM, 4>;
+    case 4: return k_noise<KIND, OUTDIM, 4>;
     }
     return nullptr;
 }
@@ -174,7 +177,7 @@ template<int KIND, int OUTDIM> static NoiseKernel pick_deriv(int indim)
     case 1: return k_noise_d<KIND, OUTDIM, 1>;
     case 2: return k_noise_d<KIND, OUTDIM, 2>;
     case 3: return k_noise_d<KIND, OUTDIM, 3>;
-    case 4: return KIND >= kSimplex ? nullptr : k_noise_d<KIND, OUTDIM, 4>;
+    case 4: return k_noise_d<KIND, OUTDIM, 4>;
     }
     return nullptr;
 }

@@ -97,6 +97,17 @@ OSLMI_DEV Df operator/(const Df& a, const Df& b)
     float ab   = a.v / b.v;
     return Df(ab, binv * (a.dx - ab * b.dx), binv * (a.dy - ab * b.dy));
 }
+OSLMI_DEV Df operator/(float a, const Df& b)
+{
+    float binv = 1.0f / b.v;
+    float ab   = a / b.v;
+    return Df(ab, binv * (-ab * b.dx), binv * (-ab * b.dy));
+}
+OSLMI_DEV Df operator/(const Df& a, float b)
+{
+    float binv = 1.0f / b;
+    return Df(a.v / b, binv * a.dx, binv * a.dy);
+}
 OSLMI_DEV Df comp(const Dv& a, int i)
 {
     return i == 0 ? Df(a.v.x, a.dx.x, a.dy.x)

@@ -240,4 +240,167 @@ D2<V3> xyz_to_rgb(const ColorSys& cs, const D2<V3>& c)
 
 float luminance(const ColorSys& cs, V3 c) { return dot(c, cs.lum_scale); }
 
+// ---- color spaces (opcolor_impl.h:246-490, Foley & van Dam) ---------------
+namespace {
+
+V3 hsv2rgb(V3 hsv)
+{
+    float h = hsv.x, s = hsv.y, v = hsv.z;
+    if (s < 0.0001f)
+        return V3(v, v, v);
+    h       = 6.0f * (h - std::floor(h));
+    int hi  = (int)std::floor(h);
+    float f = h - (float)hi;
+    float p = v * (1.0f - s);
+    float q = v * (1.0f - s * f);
+    float t = v * (1.0f - s * (1.0f - f));
+    switch (hi) {
+    case 0: return V3(v, t, p);
+    case 1: return V3(q, v, p);
+    case 2: return V3(p, v, t);
+    case 3: return V3(p, q, v);
+    case 4: return V3(t, p, v);
+    default: return V3(v, p, q);
+    }
+}
+
+// min / max keeping sfm::min_val / max_val's choice on ties (sfmath.h:219-229)
+float mn(float l, float r) { return r > l ? l : r; }
+float mx(float l, float r) { return r > l ? r : l; }
+
+V3 rgb2hsv(V3 c)
+{
+    float lo = mn(c.x, mn(c.y, c.z)), hi = mx(c.x, mx(c.y, c.z));
+    float delta = hi - lo, s = 0.0f, h = 0.0f;
+    if (hi > 0.0f)
+        s = delta / hi;
+    if (s > 0.0f) {
+        // the reference's form with 1/6 folded into the constants
+        float k, x, y;
+        if (c.x >= hi) {
+            k = 0.0f / 6.0f;
+            x = c.y;
+            y = c.z;
+        } else if (c.y >= hi) {
+            k = 2.0f / 6.0f;
+            x = c.z;
+            y = c.x;
+        } else {
+            k = 4.0f / 6.0f;
+            x = c.x;
+            y = c.y;
+        }
+        h = k + (x - y) / (6.0f * delta);
+        if (h < 0.0f)
+            h += 1.0f;
+    }
+    return V3(h, s, hi);
+}
+
+V3 hsl2rgb(V3 hsl)
+{
+    float h = hsl.x, s = hsl.y, l = hsl.z;
+    float v = (l <= 0.5f) ? (l * (1.0f + s)) : (l * (1.0f - s) + s);
+    if (v <= 0.0f)
+        return V3(0.0f);
+    float lo = 2.0f * l - v;
+    return hsv2rgb(V3(h, (v - lo) / v, v));
+}
+
+V3 rgb2hsl(V3 c)
+{
+    float lo  = mn(c.x, mn(c.y, c.z));
+    V3 hsv    = rgb2hsv(c);
+    float hi  = hsv.z;
+    float l   = (lo + hi) / 2.0f;
+    float s   = 0.0f;
+    if (lo != hi)
+        s = (hi - lo) / (l <= 0.5f ? hi + lo : 2.0f - hi - lo);
+    return V3(hsv.x, s, l);
+}
+
+// row vector times a row-major 3x3
+V3 rowmul(V3 c, const float m[3][3])
+{
+    return V3(c.x * m[0][0] + c.y * m[1][0] + c.z * m[2][0],
+              c.x * m[0][1] + c.y * m[1][1] + c.z * m[2][1],
+              c.x * m[0][2] + c.y * m[1][2] + c.z * m[2][2]);
+}
+const float kYIQ2RGB[3][3] = { { 1.0000f, 1.0000f, 1.0000f },
+                               { 0.9557f, -0.2716f, -1.1082f },
+                               { 0.6199f, -0.6469f, 1.7051f } };
+const float kRGB2YIQ[3][3] = { { 0.299f, 0.596f, 0.212f },
+                               { 0.587f, -0.275f, -0.523f },
+                               { 0.114f, -0.321f, 0.311f } };
+
+V3 xyz2xyY(V3 c)
+{
+    float n   = c.x + c.y + c.z;
+    float inv = n >= 1.0e-6f ? float(1.0f / n) : float(0.0f);
+    return V3(c.x * inv, c.y * inv, c.y);
+}
+V3 xyY2xyz(V3 c)
+{
+    float Y_y = c.y > 1.0e-6f ? float(c.z / c.y) : float(0.0f);
+    return V3(Y_y * c.x, c.z, Y_y * (1.0f - c.x - c.y));
+}
+
+// OpenImageIO safe_pow: libm pow with its special cases and clamping
+float spow(float x, float y)
+{
+    if (y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f)
+        return 0.0f;
+    if (x < 0.0f && y != std::floor(y))
+        return 0.0f;
+    float r = std::pow(x, y);
+    const float big = 3.40282346638528859812e+38f;
+    return r < -big ? -big : (r > big ? big : r);
+}
+float s2lin(float x)
+{
+    return x <= 0.04045f ? x * (1.0f / 12.92f) : spow((x + 0.055f) * (1.0f / 1.055f), 2.4f);
+}
+float lin2s(float x) { return x <= 0.0031308f ? 12.92f * x : 1.055f * spow(x, 1.f / 2.4f) - 0.055f; }
+
+}  // namespace
+
+int color_space_code(const char* name, bool tc)
+{
+    const std::string n(name);
+    if (n == "rgb" || n == "RGB" || n == "Rec709" || (tc && n == "linear"))
+        return 0;
+    const char* names[] = { "hsv", "hsl", "YIQ", "XYZ", "xyY" };
+    for (int k = 0; k < 5; ++k)
+        if (n == names[k])
+            return k + 1;
+    if (tc && n == "sRGB")
+        return 6;
+    return -1;
+}
+
+V3 transformc(const ColorSys& cs, int from, int to, V3 c)
+{
+    V3 rgb = c;
+    switch (from) {
+    case 1: rgb = hsv2rgb(c); break;
+    case 2: rgb = hsl2rgb(c); break;
+    case 3: rgb = rowmul(c, kYIQ2RGB); break;
+    case 4: rgb = rowmul(c, cs.xyz2rgb); break;
+    case 5: rgb = rowmul(xyY2xyz(c), cs.xyz2rgb); break;
+    case 6: rgb = V3(s2lin(c.x), s2lin(c.y), s2lin(c.z)); break;
+    default: break;
+    }
+    switch (to) {
+    case 1: return rgb2hsv(rgb);
+    case 2: return rgb2hsl(rgb);
+    case 3: return rowmul(rgb, kRGB2YIQ);
+    case 4: return rowmul(rgb, cs.rgb2xyz);
+    case 5: return xyz2xyY(rowmul(rgb, cs.rgb2xyz));
+    case 6: return V3(lin2s(rgb.x), lin2s(rgb.y), lin2s(rgb.z));
+    default: return rgb;
+    }
+}
+
 }  // namespace oref

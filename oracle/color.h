@@ -24,4 +24,11 @@ V3 xyz_to_rgb(const ColorSys& cs, V3 c);             // C * m_XYZ2RGB
 D2<V3> xyz_to_rgb(const ColorSys& cs, const D2<V3>& c);
 float luminance(const ColorSys& cs, V3 c);            // opcolor.h:58
 
+// Color-space names known without OpenColorIO (opcolor.cpp:275-380):
+// 0 rgb (also "RGB", the system's own name and, for transformc, "linear"),
+// 1 hsv, 2 hsl, 3 YIQ, 4 XYZ, 5 xyY, 6 sRGB (transformc only); -1 otherwise.
+int color_space_code(const char* name, bool transformc);
+// ColorSystem::transformc (opcolor.cpp:324-380) and to_rgb (:275-296)
+V3 transformc(const ColorSys& cs, int from, int to, V3 c);
+
 }  // namespace oref

@@ -600,12 +600,129 @@ static float simplex3(float x, float y, float z, int seed, float* dnoise)
     return noise;
 }
 
+// 4D: simplexnoise4 (sfm_simplex.h:970-1230) with grad4 (:440-470) over the
+// 32 tesseract edges (:184-193) and the simplex corner order of the
+// 64-entry rank table (:255-263; entry c holds each coordinate's rank,
+// 3 = largest).
+static const float kGrad4[32][4] = {
+    { 0, 1, 1, 1 },  { 0, 1, 1, -1 },  { 0, 1, -1, 1 },  { 0, 1, -1, -1 },
+    { 0, -1, 1, 1 }, { 0, -1, 1, -1 }, { 0, -1, -1, 1 }, { 0, -1, -1, -1 },
+    { 1, 0, 1, 1 },  { 1, 0, 1, -1 },  { 1, 0, -1, 1 },  { 1, 0, -1, -1 },
+    { -1, 0, 1, 1 }, { -1, 0, 1, -1 }, { -1, 0, -1, 1 }, { -1, 0, -1, -1 },
+    { 1, 1, 0, 1 },  { 1, 1, 0, -1 },  { 1, -1, 0, 1 },  { 1, -1, 0, -1 },
+    { -1, 1, 0, 1 }, { -1, 1, 0, -1 }, { -1, -1, 0, 1 }, { -1, -1, 0, -1 },
+    { 1, 1, 1, 0 },  { 1, 1, -1, 0 },  { 1, -1, 1, 0 },  { 1, -1, -1, 0 },
+    { -1, 1, 1, 0 }, { -1, 1, -1, 0 }, { -1, -1, 1, 0 }, { -1, -1, -1, 0 }
+};
+static const unsigned char kSimplex4[64][4] = {
+    { 0, 1, 2, 3 }, { 0, 1, 3, 2 }, { 0, 0, 0, 0 }, { 0, 2, 3, 1 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 0, 0, 0, 0 }, { 1, 2, 3, 0 }, { 0, 2, 1, 3 }, { 0, 0, 0, 0 }, { 0, 3, 1, 2 }, { 0, 3, 2, 1 },
+    { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 1, 3, 2, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 1, 2, 0, 3 }, { 0, 0, 0, 0 }, { 1, 3, 0, 2 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 2, 3, 0, 1 }, { 2, 3, 1, 0 }, { 1, 0, 2, 3 }, { 1, 0, 3, 2 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 0, 0, 0, 0 }, { 2, 0, 3, 1 }, { 0, 0, 0, 0 }, { 2, 1, 3, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 2, 0, 1, 3 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 3, 0, 1, 2 }, { 3, 0, 2, 1 },
+    { 0, 0, 0, 0 }, { 3, 1, 2, 0 }, { 2, 1, 0, 3 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 }, { 0, 0, 0, 0 },
+    { 3, 1, 0, 2 }, { 0, 0, 0, 0 }, { 3, 2, 0, 1 }, { 3, 2, 1, 0 }
+};
+
+static float simplex4(float x, float y, float z, float w, int seed, float* dnoise)
+{
+    const float F4 = 0.309016994f;
+    const float G4 = 0.138196601f;
+    float s        = (x + y + z + w) * F4;
+    float xs = x + s, ys = y + s, zs = z + s, ws = w + s;
+    int i = ifloor(xs), j = ifloor(ys), k = ifloor(zs), l = ifloor(ws);
+    float t  = (float)(i + j + k + l) * G4;
+    float x0 = x - ((float)i - t);
+    float y0 = y - ((float)j - t);
+    float z0 = z - ((float)k - t);
+    float w0 = w - ((float)l - t);
+    int c = ((x0 > y0) ? 32 : 0) | ((x0 > z0) ? 16 : 0) | ((y0 > z0) ? 8 : 0)
+            | ((x0 > w0) ? 4 : 0) | ((y0 > w0) ? 2 : 0) | ((z0 > w0) ? 1 : 0);
+    const unsigned char* r = kSimplex4[c];
+    OREF_COUNT(30);  // corner ordering: six compares, rank table, offsets
+    int off[3][4];
+    for (int q = 0; q < 3; ++q)
+        for (int a = 0; a < 4; ++a)
+            off[q][a] = r[a] >= 3 - q ? 1 : 0;
+    // corner offsets in (x, y, z, w): corner q is the origin minus its
+    // integer offset plus q * G4
+    float cx[5], cy[5], cz[5], cw[5];
+    cx[0] = x0;
+    cy[0] = y0;
+    cz[0] = z0;
+    cw[0] = w0;
+    const float gq[3] = { G4, 2.0f * G4, 3.0f * G4 };
+    for (int q = 0; q < 3; ++q) {
+        cx[q + 1] = x0 - (float)off[q][0] + gq[q];
+        cy[q + 1] = y0 - (float)off[q][1] + gq[q];
+        cz[q + 1] = z0 - (float)off[q][2] + gq[q];
+        cw[q + 1] = w0 - (float)off[q][3] + gq[q];
+    }
+    cx[4] = x0 - 1.0f + 4.0f * G4;
+    cy[4] = y0 - 1.0f + 4.0f * G4;
+    cz[4] = z0 - 1.0f + 4.0f * G4;
+    cw[4] = w0 - 1.0f + 4.0f * G4;
+    const int ci[5][4] = { { i, j, k, l },
+                           { i + off[0][0], j + off[0][1], k + off[0][2], l + off[0][3] },
+                           { i + off[1][0], j + off[1][1], k + off[1][2], l + off[1][3] },
+                           { i + off[2][0], j + off[2][1], k + off[2][2], l + off[2][3] },
+                           { i + 1, j + 1, k + 1, l + 1 } };
+    float g[5][4], tt[5], t2[5], t4[5], nn[5];
+    for (int q = 0; q < 5; ++q) {
+        OREF_COUNT(12);  // corner index adds, table decode (4 components)
+        uint32_t h = scramble((uint32_t)ci[q][0], (uint32_t)ci[q][1],
+                              scramble((uint32_t)ci[q][2], (uint32_t)ci[q][3], (uint32_t)seed));
+        for (int a = 0; a < 4; ++a)
+            g[q][a] = kGrad4[h & 31][a];
+        tt[q] = 0.5f - cx[q] * cx[q] - cy[q] * cy[q] - cz[q] * cz[q] - cw[q] * cw[q];
+        t2[q] = tt[q] * tt[q];
+        t4[q] = t2[q] * t2[q];
+        nn[q] = t4[q] * (g[q][0] * cx[q] + g[q][1] * cy[q] + g[q][2] * cz[q] + g[q][3] * cw[q]);
+        if (tt[q] < 0.0f)
+            nn[q] = 0.0f;
+    }
+    const float scale = 54.0f;
+    float noise       = scale * (nn[0] + nn[1] + nn[2] + nn[3] + nn[4]);
+    if (dnoise) {
+        float d[4] = { 0, 0, 0, 0 };
+        for (int q = 0; q < 5; ++q) {
+            if (tt[q] < 0.0f)
+                g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0f;
+            float temp = t2[q] * tt[q]
+                         * (g[q][0] * cx[q] + g[q][1] * cy[q] + g[q][2] * cz[q] + g[q][3] * cw[q]);
+            if (q == 0) {
+                d[0] = temp * cx[q];
+                d[1] = temp * cy[q];
+                d[2] = temp * cz[q];
+                d[3] = temp * cw[q];
+            } else {
+                d[0] += temp * cx[q];
+                d[1] += temp * cy[q];
+                d[2] += temp * cz[q];
+                d[3] += temp * cw[q];
+            }
+        }
+        for (int a = 0; a < 4; ++a) {
+            d[a] *= -8.0f;
+            d[a] += t4[0] * g[0][a] + t4[1] * g[1][a] + t4[2] * g[2][a] + t4[3] * g[3][a]
+                    + t4[4] * g[4][a];
+            dnoise[a] = d[a] * scale;
+        }
+    }
+    return noise;
+}
+
 float simplex_f(int dim, const float* p, int seed, float* dnoise)
 {
     switch (dim) {
     case 1: return simplex1(p[0], seed, dnoise);
     case 2: return simplex2(p[0], p[1], seed, dnoise);
-    default: return simplex3(p[0], p[1], p[2], seed, dnoise);
+    case 3: return simplex3(p[0], p[1], p[2], seed, dnoise);
+    default: return simplex4(p[0], p[1], p[2], p[3], seed, dnoise);
     }
 }
 
@@ -613,10 +730,13 @@ float simplex_f(int dim, const float* p, int seed, float* dnoise)
 // derivatives by the chain rule  dn/dP . dP/dx  summed left to right.
 D2<float> simplex_df(int dim, const D2<float>* p, int seed)
 {
-    float v[3] = { 0, 0, 0 }, d[3] = { 0, 0, 0 };
+    float v[4] = { 0, 0, 0, 0 }, d[4] = { 0, 0, 0, 0 };
     for (int i = 0; i < dim; ++i)
         v[i] = p[i].v;
     float r = simplex_f(dim, v, seed, d);
+    if (dim == 4)
+        return D2<float>(r, d[0] * p[0].dx + d[1] * p[1].dx + d[2] * p[2].dx + d[3] * p[3].dx,
+                         d[0] * p[0].dy + d[1] * p[1].dy + d[2] * p[2].dy + d[3] * p[3].dy);
     if (dim == 1)
         return D2<float>(r, d[0] * p[0].dx, d[0] * p[0].dy);
     if (dim == 2)

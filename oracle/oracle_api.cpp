@@ -71,8 +71,6 @@ int oref_noise(const char* name, int outdim, int indim, int derivs,
     NoiseKind kind = noise_kind(name);
     if (kind == kBad || indim < 1 || indim > 4 || (outdim != 1 && outdim != 3))
         return -1;
-    if ((kind == kSimplex || kind == kUSimplex) && indim > 3)
-        return -1;
     // cellnoise and hashnoise have no derivatives (forced off, :6109-6116)
     bool dv = derivs && kind != kCell && kind != kHash;
     for (int64_t i = 0; i < n; ++i) {

@@ -105,6 +105,18 @@ inline D2<float> operator/(const D2<float>& a, const D2<float>& b)
     float ab   = a.v / b.v;
     return D2<float>(ab, binv * (a.dx - ab * b.dx), binv * (a.dy - ab * b.dy));
 }
+// scalar / dual and dual / scalar (dual.h:592-618)
+inline D2<float> operator/(float a, const D2<float>& b)
+{
+    float binv = 1.0f / b.v;
+    float ab   = a / b.v;
+    return D2<float>(ab, binv * (-ab * b.dx), binv * (-ab * b.dy));
+}
+inline D2<float> operator/(const D2<float>& a, float b)
+{
+    float binv = 1.0f / b;
+    return D2<float>(a.v / b, binv * a.dx, binv * a.dy);
+}
 
 #ifdef OREF_F64
 // FP64 evaluation build: hashes still see the binary32 value of a

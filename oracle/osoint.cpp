@@ -506,6 +506,32 @@ struct Exec {
             setF(L, r, o == "add" ? x + y : o == "sub" ? x - y : o == "mul" ? x * y : x / y);
             return;
         }
+        if (o == "color" || o == "point" || o == "vector" || o == "normal") {
+            // triple constructors; color(space, a, b, c) converts to rgb
+            // (ColorSystem::to_rgb, opcolor.cpp:275-296)
+            const int n = (int)op.args.size();
+            if (n == 2) {
+                setT(L, r, T(L, a(1)));
+                return;
+            }
+            const int first = n == 5 ? 2 : 1;
+            const V3 c(F(L, a(first)), F(L, a(first + 1)), F(L, a(first + 2)));
+            if (n == 5 && o == "color") {
+                const int fc = color_space_code(Sv(L, a(1)).c_str(), false);
+                if (fc < 0) {
+                    fprintf(stderr, "osoint: color space %s needs OpenColorIO\n", Sv(L, a(1)).c_str());
+                    abort();
+                }
+                setT(L, r, transformc(G.cs, fc, 0, c));
+                return;
+            }
+            if (n == 5) {
+                fprintf(stderr, "osoint: %s(space, ...) is not supported\n", o.c_str());
+                abort();
+            }
+            setT(L, r, c);
+            return;
+        }
         if (o == "mod") {
             // int: a % b; float: the exact remainder with a's sign (LLVM
             // frem); 0 when b is 0 (batched_llvm_gen.cpp:1683-1750)
@@ -628,10 +654,14 @@ struct Exec {
             } else {
                 to = Sv(L, a(1));
             }
-            V3 c = T(L, a(ci_));
-            if (from == "XYZ" && (to == "rgb" || to == "RGB"))
-                c = xyz_to_rgb(G.cs, c);
-            setT(L, r, c);
+            const int fc = color_space_code(from.c_str(), true);
+            const int tc = color_space_code(to.c_str(), true);
+            if (fc < 0 || tc < 0) {
+                fprintf(stderr, "osoint: transformc %s -> %s needs OpenColorIO\n", from.c_str(),
+                        to.c_str());
+                abort();
+            }
+            setT(L, r, transformc(G.cs, fc, tc, T(L, a(ci_))));
             return;
         }
         if (o == "matrix") {

@@ -227,6 +227,26 @@ void oref_blackbody(const float* T, float* rgb, int64_t n)
     }
 }
 
+// transformc(from, to, c) over n colors (3 planes in, 3 out) by space name
+// (to_rgb's name set when to is NULL: the color(space, ...) constructor);
+// -1 for a space only OpenColorIO knows
+int oref_transformc(const char* from, const char* to, const float* in, float* out, int64_t n)
+{
+    ColorSys cs;
+    colorsys_init(cs, "Rec709");
+    const int fc = color_space_code(from, to != nullptr);
+    const int tc = to ? color_space_code(to, true) : 0;
+    if (fc < 0 || tc < 0)
+        return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        V3 c           = transformc(cs, fc, tc, V3(in[i], in[n + i], in[2 * n + i]));
+        out[i]         = c.x;
+        out[n + i]     = c.y;
+        out[2 * n + i] = c.z;
+    }
+    return 0;
+}
+
 // Table and matrices of the Rec709 color system: 317*3 table floats,
 // 9 xyz2rgb, 9 rgb2xyz, 3 luminance scale, 1 entry count.
 void oref_colorsys(const char* name, float* out)

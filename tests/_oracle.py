@@ -99,6 +99,21 @@ def blackbody(T):
     return out
 
 
+def transformc(frm, to, c):
+    """ColorSystem::transformc(from, to, c) over 3 planes (to None: the
+    color(space, ...) constructor's to_rgb)."""
+    L = lib()
+    L.oref_transformc.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_int64]
+    c = np.ascontiguousarray(c, np.float32).reshape(3, -1)
+    out = np.zeros_like(c)
+    rc = L.oref_transformc(frm.encode(), None if to is None else to.encode(), _p(c), _p(out),
+                           c.shape[1])
+    if rc != 0:
+        raise ValueError("color space needs OpenColorIO: %s -> %s" % (frm, to))
+    return out
+
+
 def colorsys(name="Rec709"):
     L = _setup_more()
     out = np.zeros(973, np.float32)

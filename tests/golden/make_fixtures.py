@@ -7,6 +7,7 @@ Reads (as data, nothing is executed):
                                       test_hash (:86-365, eps 1e-3 at :28)
   testsuite/hash/ref/out.txt          exact hash() ints (shader test.osl,
                                       testshade -g 2 2 -center)
+  testsuite/transformc/ref/out.txt    color-space transforms, 3 decimals
   testsuite/<noise tests>/ref/*.tif   uint8 reference images (512x512 RGB)
 
 Run in a container that has the reference mounted at /root/reference:
@@ -53,6 +54,29 @@ def parse_hash_out():
     return rows
 
 
+def parse_transformc_out():
+    """testsuite/transformc/ref/out.txt (shader test.osl, testshade 1x1:
+    u = v = 0.5, Dx(u) = Dy(v) = 1), printed with %0.3f."""
+    num = r"(-?\d+\.\d+)"
+    tri = r"\s+".join([num] * 3)
+    rows = {"pairs": [], "special": []}
+    for line in open(os.path.join(REF, "testsuite/transformc/ref/out.txt")):
+        line = line.strip()
+        m = re.match(r"(\w+) " + tri + r" = (\w+) " + tri + r"$", line)
+        if m:
+            g = m.groups()
+            rows["pairs"].append({"from": g[0], "in": [float(x) for x in g[1:4]],
+                                  "to": g[4], "out": [float(x) for x in g[5:8]]})
+            continue
+        m = re.match(r'transformc \("(\w+)","(\w+)",\((.*)\)\) = (.*)$', line)
+        if m:
+            frm, to, a, b = m.groups()
+            ins = [[float(x) for x in part.split()] for part in a.split("/")]
+            outs = [[float(x) for x in part.split()] for part in b.split("/")]
+            rows["special"].append({"from": frm, "to": to, "in": ins, "out": outs})
+    return rows
+
+
 def read_tif(rel):
     from PIL import Image
     im = Image.open(os.path.join(REF, rel))
@@ -65,8 +89,13 @@ def main():
                       "hash_out": "testsuite/hash/ref/out.txt"}}
     with open(os.path.join(HERE, "noise_vectors.json"), "w") as f:
         json.dump(vec, f, indent=1)
+    tc = parse_transformc_out()
+    tc["source"] = "testsuite/transformc/ref/out.txt"
+    with open(os.path.join(HERE, "transformc.json"), "w") as f:
+        json.dump(tc, f, indent=1)
     imgs = {}
-    for name in ("noise", "cellnoise", "hashnoise", "noise-simplex", "noise-gabor"):
+    for name in ("noise", "cellnoise", "hashnoise", "noise-simplex", "noise-gabor",
+                 "noise-gabor2d-filter", "pnoise-gabor"):
         imgs[name.replace("-", "_")] = read_tif("testsuite/%s/ref/out.tif" % name)
     np.savez_compressed(os.path.join(HERE, "noise_images.npz"), **imgs)
     print("wrote", sorted(os.listdir(HERE)))

@@ -129,7 +129,7 @@ def test_perlin_dual_matches_finite_difference():
 def test_simplex_dual_matches_finite_difference():
     rng = np.random.default_rng(8)
     n = 256
-    for dim in (1, 2, 3):
+    for dim in (1, 2, 3, 4):
         p = rng.uniform(-10, 10, size=(dim, n)).astype(np.float32)
         dpdx = rng.uniform(-1, 1, size=(dim, n)).astype(np.float32)
         inp = np.concatenate([p, dpdx, np.zeros_like(p)])
@@ -198,6 +198,42 @@ def test_reference_images(test, noisename, offset, scale, maxfrac):
     assert diff.max() <= 1 and np.mean(diff > 0) < 1e-4
 
 
+@pytest.mark.parametrize("image,noisename,offset,scale", [
+    ("noise_simplex", "simplex", 0.5, 0.5),  # run.py: testnoise, noisename simplex
+])
+def test_simplex_reference_image(image, noisename, offset, scale):
+    """testsuite/noise-simplex/ref/out.tif: testnoise.osl with the signed
+    simplex noise, float rows (top) and color rows (bottom) in 1-4
+    dimensions, Cout = 0.5 + 0.5 * noise. The 1D/2D/3D/4D columns pin the
+    gradient tables, the scramble seeds (0, 1, 2 for the color channels)
+    and the corner ordering of each dimension."""
+    ref = np.load(os.path.join(HERE, "golden", "noise_images.npz"))[image].astype(np.int32)
+    got = _to_u8(_render(noisename, offset, scale))
+    diff = np.abs(got - ref)
+    # runtest.py for this test: failthresh 0.004 (~1 LSB), failpercent 0.05
+    U, V, _, _, _, _ = _testnoise_coords(512)
+    for dim, cols in ((1, U < 0.24), (2, (U > 0.26) & (U < 0.49)), (3, (U > 0.51) & (U < 0.74)),
+                      (4, U > 0.76)):
+        for rows in (V < 0.49, V > 0.51):
+            d = diff[cols & rows]
+            assert d.max() <= 1 and np.mean(d > 0) < 1e-3, (dim, d.max(), np.mean(d > 0))
+
+
+def test_simplex_seeds_and_dimensions_are_distinct():
+    """A transposed seed or a dimension falling back to another would make
+    channels or columns coincide everywhere; distinct ones agree only where
+    the few lattice gradients happen to coincide (at most a few percent)."""
+    rng = np.random.default_rng(2)
+    for dim in (1, 2, 3, 4):
+        p = rng.uniform(-20, 20, (dim, 2000)).astype(np.float32)
+        v = O.noise("simplex", 3, p)
+        for a, b in ((0, 1), (0, 2), (1, 2)):
+            assert np.mean(v[a] == v[b]) < 0.05
+        if dim > 1:
+            lower = O.noise("simplex", 1, p[:dim - 1])
+            assert np.mean(lower[0] == v[0]) < 0.05
+
+
 def test_gabor_reference_image():
     """testsuite/noise-gabor (testnoise.osl with noisename gabor, default
     NoiseParams: isotropic, filtered, bandwidth 1, impulses 16). The float
@@ -223,3 +259,36 @@ def test_gabor_reference_image():
         diff = np.abs(got - ref[..., 0][m])
         # measured: exact except one pixel (1 LSB) in the filtered 2D column
         assert diff.max() <= 1 and np.mean(diff > 0) < 1e-4, (dim, diff.max())
+
+
+def test_gabor2d_filter_reference_image():
+    """testsuite/noise-gabor2d-filter: p = P * 5 * (1 + 10 v^2) * (1 + 10 u^2)
+    with testshade's P = (u, v, 1), u = x / 511, dPdx = (1/511, 0, 0),
+    dPdy = (0, 1/511, 0); Cout = 0.5 + 0.5 * noise("gabor", p[0], p[1])
+    (isotropic, filtered). The footprint grows 121-fold across the image,
+    so this pins the filter's setup and convolution over a wide range. The
+    derivatives of p follow from the product rule (they only size the
+    filter)."""
+    f32 = np.float32
+    res = 512
+    u = (np.arange(res, dtype=f32) / f32(res - 1)).astype(f32)
+    U, V = np.meshgrid(u, u)
+    su = (f32(1.0) + (f32(10) * U) * U).astype(f32)
+    sv = (f32(1.0) + (f32(10) * V) * V).astype(f32)
+    p0 = (((U * f32(5)).astype(f32) * sv).astype(f32) * su).astype(f32)
+    p1 = (((V * f32(5)).astype(f32) * sv).astype(f32) * su).astype(f32)
+    U64, V64, du = U.astype(np.float64), V.astype(np.float64), 1.0 / (res - 1)
+    dp0du = 5 * (1 + 10 * V64 ** 2) * (1 + 30 * U64 ** 2)
+    dp0dv = 5 * U64 * (20 * V64) * (1 + 10 * U64 ** 2)
+    dp1du = 5 * V64 * (1 + 10 * V64 ** 2) * (20 * U64)
+    dp1dv = 5 * (1 + 30 * V64 ** 2) * (1 + 10 * U64 ** 2)
+    inp = np.stack([p0.ravel(), p1.ravel(), (dp0du * du).ravel(), (dp1du * du).ravel(),
+                    (dp0dv * du).ravel(), (dp1dv * du).ravel()]).astype(f32)
+    r = O.gabor(inp, anisotropic=0, do_filter=1)
+    got = _to_u8((r[0] * f32(0.5) + f32(0.5)).reshape(res, res))
+    ref = np.load(os.path.join(HERE, "golden", "noise_images.npz"))["noise_gabor2d_filter"]
+    diff = np.abs(got - ref[..., 0].astype(np.int32))
+    # runtest.py: failthresh 0.004 (1 LSB), failpercent 0.05 %. Measured:
+    # 46 pixels differ by 1 LSB and 2 by 2 LSB (0.0008 %)
+    assert np.mean(diff > 1) <= 0.0005 and diff.max() <= 2, (diff.max(), np.mean(diff > 1))
+    assert np.mean(diff > 0) < 5e-4
