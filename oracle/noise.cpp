@@ -242,7 +242,32 @@ template<class T> static T perlin2(const LatHash& hash, const T& x, const T& y)
     return 0.6616f * r;
 }
 
-template<class T>
+// CGDefault (OIIO SIMD) interpolation order of the 3D/4D perlin, the
+// implementation the batched path switches to for lanes of a batch with
+// fewer active lanes than its SIMD threshold (wide_opnoise_impl_xmacro.h:
+// 57-127, thresholds wide_opnoise_perlin_float.cpp:17-26): corners packed
+// as (X,Y) (X+1,Y) (X,Y+1) (X+1,Y+1) per z plane, interpolated along z
+// first, then x, then y (oslnoise.h:834-841, :1384-1430, :1461-1500). 1D
+// and 2D interpolate in the scalar order either way.
+template<class T, class Q>
+static inline T trilerp_zxy(const T& v0, const T& v1, const T& v2, const T& v3, const T& v4,
+                            const T& v5, const T& v6, const T& v7, const Q& s, const Q& t,
+                            const Q& r)
+{
+    T xy0 = lerp(v0, v4, r), xy1 = lerp(v1, v5, r), xy2 = lerp(v2, v6, r), xy3 = lerp(v3, v7, r);
+    T xx0 = lerp(xy0, xy1, s), xx2 = lerp(xy2, xy3, s);
+    return lerp(xx0, xx2, t);
+}
+template<bool SIMD, class T, class Q>
+static inline T trilerp_p(const T& v0, const T& v1, const T& v2, const T& v3, const T& v4,
+                          const T& v5, const T& v6, const T& v7, const Q& s, const Q& t,
+                          const Q& r)
+{
+    return SIMD ? trilerp_zxy(v0, v1, v2, v3, v4, v5, v6, v7, s, t, r)
+                : trilerp(v0, v1, v2, v3, v4, v5, v6, v7, s, t, r);
+}
+
+template<class T, bool SIMD = false>
 static T perlin3(const LatHash& hash, const T& x, const T& y, const T& z)
 {
     int X, Y, Z;
@@ -252,7 +277,7 @@ static T perlin3(const LatHash& hash, const T& x, const T& y, const T& z)
     T u  = fade(fx);
     T v  = fade(fy);
     T w  = fade(fz);
-    T r  = trilerp(grad(hash(X, Y, Z), fx, fy, fz),
+    T r  = trilerp_p<SIMD>(grad(hash(X, Y, Z), fx, fy, fz),
                    grad(hash(X + 1, Y, Z), fx - 1.0f, fy, fz),
                    grad(hash(X, Y + 1, Z), fx, fy - 1.0f, fz),
                    grad(hash(X + 1, Y + 1, Z), fx - 1.0f, fy - 1.0f, fz),
@@ -265,7 +290,7 @@ static T perlin3(const LatHash& hash, const T& x, const T& y, const T& z)
     return 0.9820f * r;
 }
 
-template<class T>
+template<class T, bool SIMD = false>
 static T perlin4(const LatHash& hash, const T& x, const T& y, const T& z,
                  const T& w)
 {
@@ -278,7 +303,7 @@ static T perlin4(const LatHash& hash, const T& x, const T& y, const T& z,
     T v  = fade(fy);
     T t  = fade(fz);
     T s  = fade(fw);
-    T a  = trilerp(grad(hash(X, Y, Z, W), fx, fy, fz, fw),
+    T a  = trilerp_p<SIMD>(grad(hash(X, Y, Z, W), fx, fy, fz, fw),
                    grad(hash(X + 1, Y, Z, W), fx - 1.0f, fy, fz, fw),
                    grad(hash(X, Y + 1, Z, W), fx, fy - 1.0f, fz, fw),
                    grad(hash(X + 1, Y + 1, Z, W), fx - 1.0f, fy - 1.0f, fz, fw),
@@ -288,7 +313,7 @@ static T perlin4(const LatHash& hash, const T& x, const T& y, const T& z,
                    grad(hash(X + 1, Y + 1, Z + 1, W), fx - 1.0f, fy - 1.0f,
                         fz - 1.0f, fw),
                    u, v, t);
-    T b = trilerp(grad(hash(X, Y, Z, W + 1), fx, fy, fz, fw - 1.0f),
+    T b = trilerp_p<SIMD>(grad(hash(X, Y, Z, W + 1), fx, fy, fz, fw - 1.0f),
                   grad(hash(X + 1, Y, Z, W + 1), fx - 1.0f, fy, fz, fw - 1.0f),
                   grad(hash(X, Y + 1, Z, W + 1), fx, fy - 1.0f, fz, fw - 1.0f),
                   grad(hash(X + 1, Y + 1, Z, W + 1), fx - 1.0f, fy - 1.0f, fz,
@@ -304,15 +329,25 @@ static T perlin4(const LatHash& hash, const T& x, const T& y, const T& z,
     return 0.8344f * lerp(a, b, s);
 }
 
-template<class T> static T perlin_n(int dim, const T* p, int shift)
+template<class T, bool SIMD = false> static T perlin_n(int dim, const T* p, int shift)
 {
     LatHash h { shift };
     switch (dim) {
     case 1: return perlin1(h, p[0]);
     case 2: return perlin2(h, p[0], p[1]);
-    case 3: return perlin3(h, p[0], p[1], p[2]);
-    default: return perlin4(h, p[0], p[1], p[2], p[3]);
+    case 3: return perlin3<T, SIMD>(h, p[0], p[1], p[2]);
+    default: return perlin4<T, SIMD>(h, p[0], p[1], p[2], p[3]);
     }
+}
+
+// the CGDefault (SIMD-order) signed perlin, value and Dual2, per component
+float snoise_f_cgdefault(int dim, const float* p, int comp)
+{
+    return perlin_n<float, true>(dim, p, 8 * comp);
+}
+D2<float> snoise_df_cgdefault(int dim, const D2<float>* p, int comp)
+{
+    return perlin_n<D2<float>, true>(dim, p, 8 * comp);
 }
 
 float snoise_f(int dim, const float* p) { return perlin_n<float>(dim, p, 0); }

@@ -100,6 +100,11 @@ float snoise_f(int dim, const float* p);
 V3 snoise_v(int dim, const float* p);
 D2<float> snoise_df(int dim, const D2<float>* p);
 D2<V3> snoise_dv(int dim, const D2<float>* p);
+// The same noise in the interpolation order of the CGDefault (SIMD)
+// implementation the batched path uses for sparsely active batches;
+// component comp (0..2) of a vector result.
+float snoise_f_cgdefault(int dim, const float* p, int comp);
+D2<float> snoise_df_cgdefault(int dim, const D2<float>* p, int comp);
 // Unsigned ("uperlin"/"noise") = 0.5*(snoise+1) (oslnoise.h:2284-2400).
 float unoise_f(int dim, const float* p);
 V3 unoise_v(int dim, const float* p);

@@ -64,6 +64,35 @@ extern "C" {
 uint32_t oref_inthash(const uint32_t* k, int n) { return inthash_n(k, n); }
 uint32_t oref_bjfinal(uint32_t a, uint32_t b, uint32_t c) { return bjfinal(a, b, c); }
 
+// Signed perlin in the CGDefault (SIMD) interpolation order over SoA planes
+// (outdim 1 or 3, indim 1..4, Dual2 planes when derivs).
+int oref_perlin_cgdefault(int outdim, int indim, int derivs, const float* in, float* out,
+                          int64_t n)
+{
+    if (indim < 1 || indim > 4 || (outdim != 1 && outdim != 3))
+        return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!derivs) {
+            float p[4];
+            for (int c = 0; c < indim; ++c)
+                p[c] = in[c * n + i];
+            for (int c = 0; c < outdim; ++c)
+                out[c * n + i] = snoise_f_cgdefault(indim, p, c);
+        } else {
+            D2<float> p[4];
+            for (int c = 0; c < indim; ++c)
+                p[c] = D2<float>(in[c * n + i], in[(indim + c) * n + i], in[(2 * indim + c) * n + i]);
+            for (int c = 0; c < outdim; ++c) {
+                D2<float> r                   = snoise_df_cgdefault(indim, p, c);
+                out[c * n + i]                = r.v;
+                out[(outdim + c) * n + i]     = r.dx;
+                out[(2 * outdim + c) * n + i] = r.dy;
+            }
+        }
+    }
+    return 0;
+}
+
 // Generic noise over SoA planes. Returns 0 on success, -1 on a bad request.
 int oref_noise(const char* name, int outdim, int indim, int derivs,
                const float* in, float* out, int64_t n)

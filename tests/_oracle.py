@@ -58,6 +58,19 @@ def noise(name, outdim, inp, derivs=False):
     return out
 
 
+def perlin_cgdefault(outdim, inp, derivs=False):
+    """Signed perlin in the CGDefault (SIMD) interpolation order."""
+    L = lib()
+    L.oref_perlin_cgdefault.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    inp = np.ascontiguousarray(inp, dtype=np.float32)
+    indim = inp.shape[0] // (3 if derivs else 1)
+    out = np.zeros(((3 if derivs else 1) * outdim, inp.shape[1]), np.float32)
+    assert L.oref_perlin_cgdefault(outdim, indim, 1 if derivs else 0, _p(inp), _p(out),
+                                   inp.shape[1]) == 0
+    return out
+
+
 def hash_op(inp):
     inp = np.ascontiguousarray(inp, dtype=np.float32)
     out = np.zeros(inp.shape[1], dtype=np.int32)

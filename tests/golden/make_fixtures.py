@@ -9,6 +9,8 @@ Reads (as data, nothing is executed):
                                       testshade -g 2 2 -center)
   testsuite/transformc/ref/out.txt    color-space transforms, 3 decimals
   testsuite/<noise tests>/ref/*.tif   uint8 reference images (512x512 RGB)
+  testsuite/blackbody/ref/out.exr     half RGB image 1000x64 (ZIP-compressed
+                                      OpenEXR, decoded here)
 
 Run in a container that has the reference mounted at /root/reference:
     python tests/golden/make_fixtures.py
@@ -77,6 +79,58 @@ def parse_transformc_out():
     return rows
 
 
+def read_exr_half(rel):
+    """A scanline OpenEXR file of HALF channels with ZIP (16 lines per
+    block) or ZIPS (1 line) compression, as float16 [H, W, C] in the file's
+    channel order (alphabetical): header attributes, the line offset table,
+    then per block zlib data whose bytes are delta-coded and split into
+    even / odd halves before the per-line, per-channel pixel rows."""
+    import struct
+    import zlib
+    d = open(os.path.join(REF, rel), "rb").read()
+    assert struct.unpack("<ii", d[:8]) == (20000630, 2)
+    pos, attrs = 8, {}
+    while True:
+        e = d.index(b"\0", pos)
+        name = d[pos:e].decode()
+        pos = e + 1
+        if not name:
+            break
+        e = d.index(b"\0", pos)
+        typ = d[pos:e].decode()
+        size = struct.unpack("<i", d[e + 1:e + 5])[0]
+        attrs[name] = (typ, d[e + 5:e + 5 + size])
+        pos = e + 5 + size
+    chl, chans = attrs["channels"][1], []
+    q = 0
+    while chl[q] != 0:
+        e = chl.index(b"\0", q)
+        chans.append((chl[q:e].decode(), struct.unpack("<i", chl[e + 1:e + 5])[0]))
+        q = e + 17
+    assert all(t == 1 for _, t in chans), "HALF channels only"
+    comp = attrs["compression"][1][0]
+    lines_per_block = {2: 1, 3: 16}[comp]
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    nblocks = (h + lines_per_block - 1) // lines_per_block
+    offsets = struct.unpack("<%dQ" % nblocks, d[pos:pos + 8 * nblocks])
+    img = np.zeros((h, w, len(chans)), np.float16)
+    for off in offsets:
+        y, size = struct.unpack("<ii", d[off:off + 8])
+        raw = np.frombuffer(zlib.decompress(d[off + 8:off + 8 + size]), np.uint8)
+        t = (np.cumsum(raw.astype(np.int64)) - 128 * np.arange(len(raw))) % 256
+        t[0] = raw[0]
+        t = t.astype(np.uint8)
+        half = (len(t) + 1) // 2
+        out = np.empty_like(t)
+        out[0::2] = t[:half]
+        out[1::2] = t[half:]
+        nl = min(lines_per_block, h - (y - y0))
+        px = out.view("<f2").reshape(nl, len(chans), w)
+        img[y - y0:y - y0 + nl] = px.transpose(0, 2, 1)
+    return img, [c for c, _ in chans]
+
+
 def read_tif(rel):
     from PIL import Image
     im = Image.open(os.path.join(REF, rel))
@@ -97,6 +151,9 @@ def main():
     for name in ("noise", "cellnoise", "hashnoise", "noise-simplex", "noise-gabor",
                  "noise-gabor2d-filter", "pnoise-gabor"):
         imgs[name.replace("-", "_")] = read_tif("testsuite/%s/ref/out.tif" % name)
+    bb, chans = read_exr_half("testsuite/blackbody/ref/out.exr")
+    assert chans == ["B", "G", "R"]
+    imgs["blackbody_rgb_half"] = bb[..., ::-1]  # R, G, B
     np.savez_compressed(os.path.join(HERE, "noise_images.npz"), **imgs)
     print("wrote", sorted(os.listdir(HERE)))
 

@@ -2,8 +2,9 @@
 by the persistent batch kernel, checked against
 
   * the oracle's OSO interpreter on the same seeded points (outputs fout,
-    cout and the closure evaluation): within 4 ulp or 1e-5 relative, with
-    an absolute floor of 1e-6 for values that cancel to about zero;
+    cout and the closure evaluation): within 4 ulp or 1e-5 relative, the
+    stated bar with no absolute floor; the fraction of bit-identical values
+    is printed per group and asserted overall;
   * each group's own standalone kernel (oslmi_execute_batch) at the full
     workload counts: bit-identical, since both run the same emitted code;
   * itself across launch shapes, graph replays and the autotuner's choice:
@@ -23,7 +24,7 @@ pytestmark = pytest.mark.gpu
 NG = 64
 
 
-def close(got, ref, ulps=4, rel=1e-5, atol=1e-6):
+def close(got, ref, ulps=4, rel=1e-5):
     got = np.asarray(got, np.float32)
     ref = np.asarray(ref, np.float32)
     gi = got.view(np.int32).astype(np.int64)
@@ -31,7 +32,7 @@ def close(got, ref, ulps=4, rel=1e-5, atol=1e-6):
     gi = np.where(gi < 0, -(gi & 0x7fffffff), gi)
     ri = np.where(ri < 0, -(ri & 0x7fffffff), ri)
     d = np.abs(got.astype(np.float64) - ref)
-    return (np.abs(gi - ri) <= ulps) | (d <= rel * np.abs(ref.astype(np.float64))) | (d <= atol) \
+    return (np.abs(gi - ri) <= ulps) | (d <= rel * np.abs(ref.astype(np.float64))) \
         | (got == ref) | (np.isnan(got) & np.isnan(ref))
 
 
@@ -78,15 +79,24 @@ def test_batch_matches_oracle_interpreter(setup):
     torch.cuda.synchronize()
     got = _cpu(outs)
     bad = []
+    same = total = 0
     for k in range(NG):
         layers, conns = setup["specs"][k][0], setup["specs"][k][1]
         ref_out, ref_ev = O.Group(layers, conns).run(host[k]["P"], host[k]["N"], host[k]["wo"],
                                                     host[k]["wi"], nthreads=8)
         ok_o = close(got[k][0], ref_out)
         ok_e = close(got[k][1], ref_ev)
+        eq = np.concatenate([(got[k][0] == ref_out) | (np.isnan(got[k][0]) & np.isnan(ref_out)),
+                             (got[k][1] == ref_ev) | (np.isnan(got[k][1]) & np.isnan(ref_ev))])
+        same += int(eq.sum())
+        total += eq.size
+        print("group %2d: %5d points, exact %.5f" % (k, counts[k], eq.mean()), flush=True)
         if not (ok_o.all() and ok_e.all()):
-            bad.append((k, int((~ok_o).sum()), int((~ok_e).sum())))
-    assert not bad, "groups with mismatches (group, outputs, evals): %s" % bad
+            i = np.argwhere(~ok_o)[:3].tolist() + np.argwhere(~ok_e)[:3].tolist()
+            bad.append((k, int((~ok_o).sum()), int((~ok_e).sum()), i))
+    print("exact-match fraction over all groups: %.6f (%d values)" % (same / total, total))
+    assert not bad, "groups with mismatches (group, outputs, evals, first): %s" % bad
+    assert same / total > 0.999
 
 
 def test_batch_matches_standalone_kernels_full_counts(setup):

@@ -81,3 +81,35 @@ def test_blackbody_matches_planck():
     # XYZ->RGB of the D65 white point maps to (1,1,1)
     w = np.array([0.3127 / 0.3291, 1.0, (1 - 0.3127 - 0.3291) / 0.3291])
     assert np.allclose(w @ M, 1.0, atol=1e-4)
+
+
+def test_blackbody_reference_image():
+    """testsuite/blackbody/ref/out.exr (testshade -g 1000 64, half RGB):
+    T = 10000 u with u = x / 999; Cout = blackbody(T) / 10000, and below
+    v = 0.5 (rows 32..63) Cout * 0.7 / luminance(Cout). Compared at the
+    image's own half precision: the oracle's value rounded to half equals
+    the reference pixel, or is its half neighbour (the oracle's own cube
+    root stands in for OpenImageIO's fast_cbrt in the table index)."""
+    import os
+    ref = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                               "noise_images.npz"))["blackbody_rgb_half"].astype(np.float32)
+    x = np.arange(1000, dtype=np.float32)
+    u = (x / np.float32(999)).astype(np.float32)
+    T = (np.float32(10000.0) * u).astype(np.float32)
+    top = (O.blackbody(T) / np.float32(10000.0)).astype(np.float32)
+    cs = O.colorsys()
+    lum = (top[0] * cs["lum"][0] + top[1] * cs["lum"][1] + top[2] * cs["lum"][2]).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        bottom = (top * (np.float32(0.7) / lum)).astype(np.float32)
+    for rows, want in ((slice(0, 32), top), (slice(32, 64), bottom)):
+        r = ref[rows]
+        assert np.all(r == r[0])  # every row of a half is the same T sweep
+        got = want.T.astype(np.float16)
+        fin = np.isfinite(r[0]) & np.isfinite(got.astype(np.float32))
+        a = got[fin].view(np.uint16).astype(np.int64)
+        b = r[0][fin].astype(np.float16).view(np.uint16).astype(np.int64)
+        assert np.abs(a - b).max() <= 1
+        # measured: 2999 of 3000 (top) half values equal, the other 1 apart
+        assert np.mean(a == b) > 0.99
+        # non-finite pixels (T = 0: luminance 0) agree too
+        np.testing.assert_array_equal(np.isfinite(r[0]), np.isfinite(got.astype(np.float32)))
