@@ -161,6 +161,87 @@ int oslmi_execute_batch_sample(oslmi_shadingsys* ss, oslmi_group* g,
                                const float* d_wo, const float* d_rnd, float* d_samples,
                                void* stream);
 
+/* ---- The reference's own batch layout ----------------------------------
+ * BatchedShaderGlobals<16> byte for byte (src/include/OSL/
+ * batched_shaderglobals.h:21-140): the 64-byte UniformShaderGlobals block,
+ * then VaryingShaderGlobals<16>, whose Block<Vec3,16> fields are x[16],
+ * y[16], z[16] (src/include/OSL/wide.h:439-445) and Block<float,16> /
+ * Block<int,16> / Block<pointer,16> 16 values each. 4032 bytes, 64-byte
+ * aligned. A renderer that fills these for the batched CPU path hands the
+ * same blocks (copied to device memory) to oslmi_execute_wide16. The
+ * uniform pointers and object2common / shader2common / Ci are opaque
+ * renderer data here and are not read. */
+#if defined(__cplusplus)
+#define OSLMI_ALIGN64 alignas(64)
+#else
+#define OSLMI_ALIGN64 _Alignas(64)
+#endif
+typedef struct oslmi_bsg16 {
+    /* UniformShaderGlobals (batched_shaderglobals.h:21-75) */
+    OSLMI_ALIGN64 void* renderstate;
+    void* tracedata;
+    void* objdata;
+    void* context;
+    void* renderer;
+    int raytype;
+    int pad[5];
+    /* VaryingShaderGlobals<16> (batched_shaderglobals.h:77-140) */
+    OSLMI_ALIGN64 float P[3][16];
+    float dPdx[3][16], dPdy[3][16], dPdz[3][16];
+    float I[3][16], dIdx[3][16], dIdy[3][16];
+    float N[3][16];
+    float Ng[3][16];
+    float u[16], dudx[16], dudy[16];
+    float v[16], dvdx[16], dvdy[16];
+    float dPdu[3][16], dPdv[3][16];
+    float time[16];
+    float dtime[16];
+    float dPdtime[3][16];
+    float Ps[3][16], dPsdx[3][16], dPsdy[3][16];
+    const void* object2common[16];
+    const void* shader2common[16];
+    void* Ci[16];
+    float surfacearea[16];
+    int flipHandedness[16];
+    int backfacing[16];
+} oslmi_bsg16;
+
+/* Where a symbol's value goes on the renderer side (SymLocationDesc,
+ * src/include/OSL/oslexec.h:69-104; ShadingSystem::add_symlocs, :1072-1076).
+ * Only the output arena is supported: point i's value is written to
+ * output_base + offset + shadeindex[i] * stride. type: "float", "int",
+ * "color", "point", "vector" or "normal"; derivs must be 0. name:
+ * "layer.symbol" or "symbol" (the last layer that has it). */
+#define OSLMI_ARENA_HEAP 0
+#define OSLMI_ARENA_OUTPUTS 1
+#define OSLMI_ARENA_USERDATA 2
+typedef struct oslmi_symloc {
+    const char* name;
+    const char* type;
+    int arena;
+    int derivs;
+    int64_t offset;
+    int64_t stride;
+} oslmi_symloc;
+/* add_symlocs(group, symlocs) / clear_symlocs(group). Symbols placed in
+ * the output arena become the group's renderer outputs: before the group
+ * is compiled they set its output list; afterwards each must already be one
+ * of its outputs. At most 16 per group. */
+int oslmi_add_symlocs(oslmi_shadingsys* ss, oslmi_group* g, int n, const oslmi_symloc* locs);
+int oslmi_clear_symlocs(oslmi_shadingsys* ss, oslmi_group* g);
+
+/* BatchedExecutor<16>::execute (src/include/OSL/oslexec.h:1000-1004) for
+ * nbatch batches at once: d_bsg nbatch BatchedShaderGlobals<16> blocks in
+ * device memory; batch b holds 16 points except the last, which holds
+ * last_batch_size (1..16, the reference's batch_size); d_shadeindex the
+ * batches' Wide<const int,16> shade indices (nbatch * 16 ints, device);
+ * outputs placed by the group's symlocs relative to d_output_base (device).
+ * Allocates a per-group device workspace on first use (the globals and
+ * outputs in structure-of-arrays form, reused by later calls). */
+int oslmi_execute_wide16(oslmi_shadingsys* ss, oslmi_group* g, int64_t nbatch,
+                         int last_batch_size, const oslmi_bsg16* d_bsg,
+                         const int32_t* d_shadeindex, void* d_output_base, void* stream);
+
 /* ---- Multi-group batches -----------------------------------------------
  * Many groups shaded in one persistent launch. The reference executes one
  * group per BatchedExecutor<W>::execute call (oslexec.h:1000-1004); a

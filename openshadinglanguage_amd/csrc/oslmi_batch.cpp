@@ -148,31 +148,15 @@ struct Batch {
     // block * ppt; within a part, items in the configured order.
     int build_tiles()
     {
-        const int64_t ts = (int64_t)cfg.block * cfg.ppt;
-        const int ni     = (int)hargs.size();
-        tiles.clear();
-        item_first.assign(ni, 0);
-        item_count.assign(ni, 0);
-        for (Part& p : parts) {
-            std::vector<int> order = p.items;
-            if (cfg.order == 1 && (int)cost.size() == ni)
-                std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-            p.first_tile = (int)tiles.size();
-            for (int k : order) {
-                item_first[k] = (int)tiles.size();
-                for (int64_t b = 0; b < hargs[k].n; b += ts) {
-                    BatchTile t;
-                    t.item  = k;
-                    t.pad   = 0;
-                    t.begin = b;
-                    t.end   = std::min<int64_t>(b + ts, hargs[k].n);
-                    tiles.push_back(t);
-                    if (tiles.size() > (size_t)0x7fffffff)
-                        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: too many tiles");
-                }
-                item_count[k] = (int)tiles.size() - item_first[k];
-            }
-            p.ntiles = (int)tiles.size() - p.first_tile;
+        std::vector<int64_t> counts(hargs.size());
+        for (size_t k = 0; k < hargs.size(); ++k)
+            counts[k] = hargs[k].n;
+        std::vector<int> part_first, part_ntiles;
+        if (int rc = tile_list(counts, tiles, item_first, item_count, part_first, part_ntiles))
+            return rc;
+        for (size_t q = 0; q < parts.size(); ++q) {
+            parts[q].first_tile = part_first[q];
+            parts[q].ntiles     = part_ntiles[q];
         }
         if (tiles.size() > dtiles_cap) {
             if (dtiles)
@@ -192,6 +176,44 @@ struct Batch {
         return OSLMI_OK;
     }
 
+    // The tile list of items with `counts` points under the current config
+    // (no device work): tiles of block * ppt points, each part's items
+    // together, in item order or by measured cost per tile.
+    int tile_list(const std::vector<int64_t>& counts, std::vector<BatchTile>& out,
+                  std::vector<int>& first, std::vector<int>& count, std::vector<int>& part_first,
+                  std::vector<int>& part_ntiles) const
+    {
+        const int64_t ts = (int64_t)cfg.block * cfg.ppt;
+        const int ni     = (int)counts.size();
+        out.clear();
+        first.assign(ni, 0);
+        count.assign(ni, 0);
+        part_first.assign(parts.size(), 0);
+        part_ntiles.assign(parts.size(), 0);
+        for (size_t q = 0; q < parts.size(); ++q) {
+            std::vector<int> order = parts[q].items;
+            if (cfg.order == 1 && (int)cost.size() == ni)
+                std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+            part_first[q] = (int)out.size();
+            for (int k : order) {
+                first[k] = (int)out.size();
+                for (int64_t b = 0; b < counts[k]; b += ts) {
+                    BatchTile t;
+                    t.item  = k;
+                    t.pad   = 0;
+                    t.begin = b;
+                    t.end   = std::min<int64_t>(b + ts, counts[k]);
+                    out.push_back(t);
+                    if (out.size() > (size_t)0x7fffffff)
+                        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: too many tiles");
+                }
+                count[k] = (int)out.size() - first[k];
+            }
+            part_ntiles[q] = (int)out.size() - part_first[q];
+        }
+        return OSLMI_OK;
+    }
+
     int part_of(int item) const
     {
         for (size_t p = 0; p < parts.size(); ++p)
@@ -205,20 +227,26 @@ struct Batch {
     // counter slot c.
     int launch_range(const Part& p, int first, int count, int c, hipStream_t stream)
     {
+        return launch_range_on(p, first, count, dargs, dtiles, dcounter + c, bb, stream);
+    }
+    int launch_range_on(const Part& p, int first, int count, const GroupArgs* d_args,
+                        const BatchTile* d_tiles, int32_t* d_counter, const float* d_bb,
+                        hipStream_t stream)
+    {
         if (count <= 0)
             return OSLMI_OK;
         int64_t grid = (int64_t)std::min(cfg.blocks_per_cu, p.max_blocks[block_slot(cfg.block)]) * num_cu;
         grid         = std::max<int64_t>(1, std::min<int64_t>(grid, count));
         BatchArgs B;
         memset(&B, 0, sizeof(B));
-        B.args       = dargs;
-        B.tiles      = dtiles;
+        B.args       = d_args;
+        B.tiles      = d_tiles;
         B.tile_begin = first;
         B.tile_end   = first + count;
-        B.counter    = dcounter + c;
+        B.counter    = d_counter;
         B.ppt        = cfg.ppt;
         B.bb_lds     = cfg.bb_lds;
-        B.bb         = bb;
+        B.bb         = d_bb;
         void* args[] = { &B };
         return check_hip(hipModuleLaunchKernel(p.fn, (unsigned)grid, 1, 1, (unsigned)cfg.block, 1, 1, 0,
                                                stream, args, nullptr),

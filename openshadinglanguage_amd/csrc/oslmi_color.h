@@ -110,16 +110,17 @@ template<class F> OSLMI_DEV F cs_min(const F& l, const F& r) { return cs_val(r) 
 template<class F> OSLMI_DEV F cs_max(const F& l, const F& r) { return cs_val(r) > cs_val(l) ? r : l; }
 
 // OpenImageIO's safe_pow: 1 for y == 0, 0 for x == 0 or a negative x with
-// a non-integer y, otherwise pow clamped to the finite range
+// a non-integer y, otherwise x^y (as 2^(log2|x| y), negated for a negative
+// x and an odd integer y) clamped to the finite range
 OSLMI_DEV float cs_safe_pow(float x, float y)
 {
     if (y == 0.0f)
         return 1.0f;
-    if (x == 0.0f)
+    if (x == 0.0f || (x < 0.0f && y != floorf(y)))
         return 0.0f;
-    if (x < 0.0f && y != floorf(y))
-        return 0.0f;
-    float r = powf(x, y);
+    float r = fexp2(flog2(fabsf(x)) * y);
+    if (x < 0.0f && fmodf(y, 2.0f) != 0.0f)
+        r = -r;
     const float big = 3.40282346638528859812e+38f;
     return r < -big ? -big : (r > big ? big : r);
 }

@@ -47,6 +47,15 @@ struct OutputSlot {
     int first_plane;
 };
 
+// A renderer-side location of an output symbol (SymLocationDesc in the
+// output arena): point i's value goes to base + offset + shadeindex * stride.
+struct SymLoc {
+    std::string name;
+    int comps  = 1;  // 1 or 3 values
+    int is_int = 0;
+    int64_t offset = 0, stride = 0;
+};
+
 struct Group {
     std::string name;
     bool ended = false;
@@ -66,6 +75,11 @@ struct Group {
     hipModule_t module       = nullptr;
     hipFunction_t kernel     = nullptr;
     double jit_seconds       = 0.0;
+    // oslmi_execute_wide16: symbol locations and the device workspace of
+    // gathered globals and output planes (grown on demand)
+    std::vector<SymLoc> symlocs;
+    float* wide_ws        = nullptr;
+    size_t wide_ws_floats = 0;
     std::mutex mu;
     int find_layer(const std::string& n) const;
 };

@@ -377,9 +377,12 @@ void oslmi_shadingsys_destroy(oslmi_shadingsys* s)
 {
     if (!s)
         return;
-    for (auto& g : s->ss.groups)
+    for (auto& g : s->ss.groups) {
         if (g->module)
             (void)hipModuleUnload(g->module);
+        if (g->wide_ws)
+            (void)hipFree(g->wide_ws);
+    }
     if (s->ss.xf.device)
         (void)hipFree(s->ss.xf.device);
     delete s;

@@ -26,9 +26,11 @@
 // two transcendentals (exp2, rcp) and two VALU operations per unit, with
 // the -log2(e) scaling folded into the packed weights, and the output
 // epilogue is one exp2 (output normalisation folded into W5 and b5).
-// Queries that are outside the training domain or in a routing cell
-// flagged by calibration are appended to a fallback list that the exact
-// closure kernel then evaluates.
+// The same kernel routes the queries as it reads them (one pass over the
+// 14 input planes): exact zeros are written at once, queries outside the
+// training domain or in a routing cell flagged by calibration are appended
+// to a fallback list that the exact closure kernel then evaluates, and the
+// rest are queued per wave in LDS until two 16-query tiles are ready.
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -402,131 +404,190 @@ __device__ __forceinline__ int route_cell(float lx, float lz, float vx, float vz
     return a * 8 + b;
 }
 
-// mode 0: network for every in-domain query (no routing table);
-// Sorts the queries: V.z > 0 and L.z <= 0 -> zeros (exact); inside the
-// training domain and (mode 1) not in a routed cell -> network list;
-// otherwise -> fallback list (mode 1) or NaN outputs (mode 0).
-// class of query i: 0 exact zero, 1 network, 2 fallback (mode 1) or not
-// answered (mode 0)
-// All 13 planes are loaded unconditionally (one batch of independent loads
-// per query, no branch between them) and the class is formed with selects.
-__device__ __forceinline__ int route_class(const Domain& dom, const unsigned char* cells, int mode,
-                                           const float* __restrict__ q, int64_t n, int64_t i)
+// Class of a query: 0 exact zero (V.z > 0 and L.z <= 0), 1 network
+// (inside the training domain and, in mode 1, not in a routed cell), 2
+// fallback (mode 1: the exact closure) or unanswered (mode 0: NaN).
+__device__ __forceinline__ int route_class(const Domain& dom, unsigned long long cell_mask, int mode,
+                                           const float v[NIN])
 {
-    float v[14];
-#pragma unroll
-    for (int k = 0; k < 14; k++)
-        v[k] = k == 1 ? 0.0f : q[k * n + i];
     const float lz = v[2], vz = v[4];
     bool inside = vz > 0.0f;
 #pragma unroll
     for (int k = 0; k < 9; k++)
         inside = inside && v[5 + k] >= dom.lo[k] && v[5 + k] <= dom.hi[k];
     const int cell = route_cell(v[0], lz, v[3], vz, v[6]);
-    if (inside && mode == 1 && cells[cell])
+    if (inside && mode == 1 && ((cell_mask >> cell) & 1ull))
         inside = false;
     if (vz > 0.0f && lz <= 0.0f)
         return 0;
     return inside ? 1 : 2;
 }
 
-constexpr int ROUTE_CHUNK = 1024;  // queries per workgroup (4 rounds of 256)
+// Per-wave queue of network queries in LDS: a ring of QCAP slots of 16
+// floats (14 raw inputs, the query index as bits, padding).
+constexpr int QCAP = 96, QSLOT = 16, WAVES_PER_BLOCK = 4;
+// Per-wave fallback list in LDS (flushed to the global list when full).
+constexpr int FBCAP = 64;
 
-// Order-preserving compaction: workgroup b owns queries [b*1024, (b+1)*1024).
-// Pass 1 classifies them (one read of the query planes; classes kept in
-// LDS) and counts network and fallback queries, one atomic per list
-// reserves the workgroup's slots, pass 2 writes the indices in query order
-// (wave ballots plus a prefix over the 4 waves of each 256-query round).
-__global__ __launch_bounds__(256) void k_route(Domain dom, const unsigned char* __restrict__ cells,
-                                               int mode, const float* __restrict__ q, int64_t n,
-                                               float* __restrict__ out, int* __restrict__ net,
-                                               int* __restrict__ netn, int* __restrict__ fb,
-                                               int* __restrict__ fbn)
+// The network over TILES 16-query tiles whose inputs sit in queue slots
+// head + 16 u + r (u < TILES, r = lane % 16) of this wave's ring; `count`
+// of them are valid. Writes the outputs (or appends to the fallback list).
+template<int TILES>
+__device__ __forceinline__ void run_tiles(const float* __restrict__ ring, int head, int count,
+                                          const float noff[8], const float nscl[8],
+                                          const bf16x8 w1[4], const bf16x8 w2[4][2],
+                                          const bf16x8 w3[4][2], const bf16x8 w4[4][2],
+                                          const bf16x8 w5[2], const f32x4* __restrict__ s_b,
+                                          int64_t n, float* __restrict__ out, bool spot_check,
+                                          float min_weight, bool spot[TILES], int qi_out[TILES])
 {
-    __shared__ int wcnt[2][4];
-    __shared__ int base[2];
-    __shared__ signed char cls[ROUTE_CHUNK];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * ROUTE_CHUNK;
-    int mine[2] = { 0, 0 };
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    bf16x8 x[TILES];
+    int qi[TILES];
 #pragma unroll
-    for (int r = 0; r < ROUTE_CHUNK / 256; r++) {
-        const int64_t i = c0 + r * 256 + threadIdx.x;
-        int c = -1;
-        if (i < n) {
-            c = route_class(dom, cells, mode, q, n, i);
-            if (c == 1)
-                mine[0]++;
-            else if (c == 2 && mode == 1)
-                mine[1]++;
-            else if (c == 0)
-                out[i] = out[n + i] = out[2 * n + i] = out[3 * n + i] = 0.0f;
-            else if (c == 2 && mode == 0)
-                out[i] = out[n + i] = out[2 * n + i] = out[3 * n + i] = __builtin_nanf("");
+    for (int u = 0; u < TILES; u++) {
+        const int e     = u * 16 + r;
+        const float* sl = ring + ((head + e) % QCAP) * QSLOT;
+        qi[u]           = e < count ? __float_as_int(sl[14]) : -1;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            // layer-1 operand: this lane group's 8 inputs (normalised)
+            const int k = 8 * g + j;
+            x[u][j] = (bf16)(k < NIN && e < count ? (sl[k < NIN ? k : 0] - noff[j]) * nscl[j] : 0.0f);
         }
-        cls[r * 256 + threadIdx.x] = (signed char)c;
     }
-    // block totals
-    for (int l = 0; l < 2; l++) {
-        int v = mine[l];
-        for (int off = 32; off > 0; off >>= 1)
-            v += __shfl_down(v, off);
-        if (lane == 0)
-            wcnt[l][wv] = v;
+    f32x4 a[TILES][4];
+    bf16x8 h0[TILES], h1[TILES];
+    f32x4 y2[TILES];
+#pragma unroll
+    for (int u = 0; u < TILES; u++)
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x[u], s_b[t * 64 + lane], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < TILES; u++) {
+        h0[u] = act_pair(a[u][0], a[u][1]);
+        h1[u] = act_pair(a[u][2], a[u][3]);
     }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        const int l = threadIdx.x;
-        const int tot = wcnt[l][0] + wcnt[l][1] + wcnt[l][2] + wcnt[l][3];
-        base[l]       = tot ? atomicAdd(l == 0 ? netn : fbn, tot) : 0;
-    }
-    __syncthreads();
-    int run[2] = { base[0], base[1] };
-    __syncthreads();
-    for (int r = 0; r < ROUTE_CHUNK / 256; r++) {
-        const int64_t i = c0 + r * 256 + threadIdx.x;
-        const int c     = cls[r * 256 + threadIdx.x];
-        unsigned long long bal[2] = { __ballot(c == 1), __ballot(c == 2 && mode == 1) };
-        if (lane == 0) {
-            wcnt[0][wv] = __popcll(bal[0]);
-            wcnt[1][wv] = __popcll(bal[1]);
+#pragma unroll
+    for (int u = 0; u < TILES; u++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], h0[u], s_b[(4 + t) * 64 + lane], 0, 0, 0);
+            a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][1], h1[u], a[u][t], 0, 0, 0);
         }
-        __syncthreads();
-        for (int l = 0; l < 2; l++) {
-            int before = 0, total = 0;
-            for (int w = 0; w < 4; w++) {
-                before += w < wv ? wcnt[l][w] : 0;
-                total += wcnt[l][w];
-            }
-            const bool take = (l == 0) ? (c == 1) : (c == 2 && mode == 1);
-            if (take) {
-                const int pos = run[l] + before + __popcll(bal[l] & ((1ull << lane) - 1ull));
-                (l == 0 ? net : fb)[pos] = (int)i;
-            }
-            run[l] += total;
+#pragma unroll
+    for (int u = 0; u < TILES; u++) {
+        h0[u] = act_pair(a[u][0], a[u][1]);
+        h1[u] = act_pair(a[u][2], a[u][3]);
+    }
+#pragma unroll
+    for (int u = 0; u < TILES; u++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], h0[u], s_b[(8 + t) * 64 + lane], 0, 0, 0);
+            a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][1], h1[u], a[u][t], 0, 0, 0);
         }
-        __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TILES; u++) {
+        h0[u] = act_pair(a[u][0], a[u][1]);
+        h1[u] = act_pair(a[u][2], a[u][3]);
+    }
+#pragma unroll
+    for (int u = 0; u < TILES; u++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][0], h0[u], s_b[(12 + t) * 64 + lane], 0, 0, 0);
+            a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][1], h1[u], a[u][t], 0, 0, 0);
+        }
+#pragma unroll
+    for (int u = 0; u < TILES; u++) {
+        h0[u] = act_pair(a[u][0], a[u][1]);
+        h1[u] = act_pair(a[u][2], a[u][3]);
+    }
+#pragma unroll
+    for (int u = 0; u < TILES; u++) {
+        y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[0], h0[u], s_b[16 * 64 + lane], 0, 0, 0);
+        y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[1], h1[u], y2[u], 0, 0, 0);
+    }
+    // lanes of group 0 hold outputs 0..3 of their query. Spot check (when
+    // spot_check): a predicted weight below the threshold has a large
+    // relative error (the log-space fit is absolute); such a query is not
+    // written but flagged in spot[u] / qi_out[u] for the exact closure.
+#pragma unroll
+    for (int u = 0; u < TILES; u++) {
+        float v[4];
+#pragma unroll
+        for (int o = 0; o < 4; o++)
+            v[o] = __builtin_amdgcn_exp2f(y2[u][o]);
+        const bool mine = g == 0 && qi[u] >= 0;
+        spot[u]   = spot_check && mine && fminf(fminf(v[0], v[1]), v[2]) < min_weight;
+        qi_out[u] = qi[u];
+        if (mine && !spot[u]) {
+#pragma unroll
+            for (int o = 0; o < 4; o++)
+                out[o * n + qi[u]] = v[o];
+        }
     }
 }
 
-// The network over the queries listed in net[0 .. *netn).
+// Routing and network in one pass over the queries. Each wave walks chunks
+// of 64 consecutive queries (lane = query): it reads the 14 input planes
+// once (coalesced), answers the exact zeros at once, appends fallback
+// queries to the fallback list (one atomic per wave and chunk, wave ballot
+// for the positions), and queues network queries in its LDS ring (ballot
+// compaction, query order kept). Whenever 32 are queued, the network runs
+// on two 16-query tiles; the rest is flushed after the last chunk. The next
+// chunk's inputs are loaded before the network runs on the current queue.
+// The weights stay in the accumulation registers for the whole kernel (the
+// MFMA reads its A operand there), the bias initialisers in LDS.
 template<int TILES>
-__global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __restrict__ frag,
-                                               const f32x4* __restrict__ biasp,
-                                               const int* __restrict__ net,
-                                               const int* __restrict__ netn,
-                                               const float* __restrict__ q, int64_t n,
-                                               float* __restrict__ out, float min_weight,
-                                               int* __restrict__ fbidx, int* __restrict__ fbcount)
+__global__ __launch_bounds__(256) void k_infer_fused(InferConsts kc, unsigned long long cell_mask,
+                                                     int mode, const bf16x8* __restrict__ frag,
+                                                     const f32x4* __restrict__ biasp,
+                                                     const float* __restrict__ q, int64_t n,
+                                                     float* __restrict__ out, float min_weight,
+                                                     int* __restrict__ fbidx, int* __restrict__ fbcount)
 {
-    // bias initialisers (MFMA C operands) in LDS, read as each layer starts;
-    // in registers they would cost 68 VGPRs per wave
     __shared__ f32x4 s_b[NBIAS * 64];
+    __shared__ float s_ring[WAVES_PER_BLOCK][QCAP * QSLOT];
+    __shared__ int s_fb[WAVES_PER_BLOCK][FBCAP];
     for (int j = threadIdx.x; j < NBIAS * 64; j += blockDim.x)
         s_b[j] = biasp[j];
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int g = lane >> 4, r = lane & 15;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g    = lane >> 4;
+    float* ring    = s_ring[wv];
+    int* fbl       = s_fb[wv];
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    // this wave's fallback list: fbn (wave-uniform) indices in fbl, moved
+    // to the global list with one atomic when full and at the end
+    int fbn         = 0;
+    auto fb_flush   = [&]() {
+        if (fbn == 0)
+            return;
+        int base = 0;
+        if (lane == 0)
+            base = atomicAdd(fbcount, fbn);
+        base = __shfl(base, 0);
+        if (lane < fbn)
+            fbidx[base + lane] = fbl[lane];
+        fbn = 0;
+        __builtin_amdgcn_wave_barrier();
+    };
+    auto fb_append = [&](bool flag, int idx) {
+        const unsigned long long m = __ballot(flag);
+        if (m == 0)
+            return;
+        const int cnt = __popcll(m);
+        if (fbn + cnt > FBCAP)
+            fb_flush();
+        if (flag)
+            fbl[fbn + __popcll(m & lt)] = idx;
+        fbn += cnt;
+        __builtin_amdgcn_wave_barrier();
+    };
     bf16x8 w1[4], w2[4][2], w3[4][2], w4[4][2], w5[2];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
@@ -540,9 +601,6 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
     }
     w5[0] = frag[28 * 64 + lane];
     w5[1] = frag[29 * 64 + lane];
-    // the weights go to the accumulation registers, which the MFMA reads its
-    // A operand from directly; the VGPRs stay free for the accumulators and
-    // the SiLU work (no accumulator-to-VGPR copies)
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         asm volatile("" : "+a"(w1[t]));
@@ -555,39 +613,6 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
     }
     asm volatile("" : "+a"(w5[0]));
     asm volatile("" : "+a"(w5[1]));
-
-    const int64_t cnt    = *netn;
-    const int64_t npairs = (cnt + 16 * TILES - 1) / (16 * TILES);
-    const int64_t wave   = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    // Two 16-query tiles per iteration (independent MFMA/SiLU chains that the
-    // scheduler interleaves; one wave per SIMD holds all the weights), and a
-    // software pipeline over iterations: the next pair's query indices and
-    // raw inputs are loaded while the current pair is computed.
-    // loads are unconditional (clamped in-bounds addresses, results selected
-    // afterwards) so that they issue back to back without branches
-    auto load_idx = [&](int64_t pair, int qi[TILES]) {
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            const int64_t e = (TILES * pair + u) * 16 + r;
-            const bool ok   = e < cnt;
-            const int v     = net[ok ? e : 0];
-            qi[u]           = v | (ok ? 0 : -1);  // -1 marks no query
-        }
-    };
-    auto load_x = [&](const int qi[TILES], float xr[TILES][8]) {
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            const int64_t qs = qi[u] >= 0 ? qi[u] : 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int k         = 8 * g + j;
-                const int kc_       = k < NIN ? k : NIN - 1;
-                const uint32_t keep = (k < NIN && qi[u] >= 0) ? 0xffffffffu : 0u;
-                xr[u][j] = __uint_as_float(__float_as_uint(q[kc_ * n + qs]) & keep);
-            }
-        }
-    };
     // this lane group's normalisation constants (inputs 8g .. 8g+7)
     float noff[8], nscl[8];
 #pragma unroll
@@ -596,112 +621,71 @@ __global__ __launch_bounds__(256) void k_infer(InferConsts kc, const bf16x8* __r
         noff[j]     = k < NIN ? kc.nm.in_off[k < NIN ? k : 0] : 0.0f;
         nscl[j]     = k < NIN ? kc.nm.in_scale[k < NIN ? k : 0] : 0.0f;
     }
-    int qcur[TILES], qnext[TILES];
-    float xcur[TILES][8];
-    load_idx(wave, qcur);
-    load_x(qcur, xcur);
-    load_idx(wave + nwaves, qnext);
-    for (int64_t pair = wave; pair < npairs; pair += nwaves) {
-        // keeps the (loop-invariant) LDS bias reads inside the loop
-        asm volatile("" ::: "memory");
-        float xnext[TILES][8];
-        int qnn[TILES];
-        load_x(qnext, xnext);
-        load_idx(pair + 2 * nwaves, qnn);
-        f32x4 y2[TILES];
-        bf16x8 x[TILES];
+    const int64_t nchunks = (n + 63) / 64;
+    const int64_t wave    = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wv;
+    const int64_t nwaves  = (int64_t)gridDim.x * WAVES_PER_BLOCK;
+    // unconditional loads (clamped in-bounds address, value unused past n)
+    auto load = [&](int64_t chunk, float v[NIN]) {
+        const int64_t i  = chunk * 64 + lane;
+        const int64_t ic = i < n ? i : n - 1;
 #pragma unroll
-        for (int u = 0; u < TILES; u++)
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                // layer-1 operand: this lane group's 8 inputs (normalised)
-                const int k = 8 * g + j;
-                x[u][j] = (bf16)(k < NIN ? (xcur[u][j] - noff[j]) * nscl[j] : 0.0f);
-            }
-        f32x4 a[TILES][4];
-        bf16x8 h0[TILES], h1[TILES];
-#pragma unroll
-        for (int u = 0; u < TILES; u++)
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[t], x[u], s_b[t * 64 + lane], 0, 0, 0);
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            h0[u] = act_pair(a[u][0], a[u][1]);
-            h1[u] = act_pair(a[u][2], a[u][3]);
+        for (int k = 0; k < NIN; k++)
+            v[k] = q[k * n + ic];
+    };
+    int head = 0, pending = 0;
+    float cur[NIN];
+    if (wave < nchunks)
+        load(wave, cur);
+    for (int64_t chunk = wave; chunk < nchunks; chunk += nwaves) {
+        const int64_t i = chunk * 64 + lane;
+        const bool live = i < n;
+        int c = live ? route_class(kc.dom, cell_mask, mode, cur) : -1;
+        if (c == 0) {
+            out[i] = out[n + i] = out[2 * n + i] = out[3 * n + i] = 0.0f;
+        } else if (c == 2 && mode == 0) {
+            out[i] = out[n + i] = out[2 * n + i] = out[3 * n + i] = __builtin_nanf("");
         }
+        if (mode == 1)
+            fb_append(c == 2, (int)i);
+        const unsigned long long nb = __ballot(c == 1);
+        if (c == 1) {
+            float* sl = ring + ((head + pending + __popcll(nb & lt)) % QCAP) * QSLOT;
 #pragma unroll
-        for (int u = 0; u < TILES; u++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][0], h0[u], s_b[(4 + t) * 64 + lane], 0, 0, 0);
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[t][1], h1[u], a[u][t], 0, 0, 0);
-            }
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            h0[u] = act_pair(a[u][0], a[u][1]);
-            h1[u] = act_pair(a[u][2], a[u][3]);
+            for (int k = 0; k < NIN; k++)
+                sl[k] = cur[k];
+            sl[14] = __int_as_float((int)i);
         }
+        pending += __popcll(nb);
+        __builtin_amdgcn_wave_barrier();
+        // the next chunk's inputs are in flight while the network runs
+        if (chunk + nwaves < nchunks)
+            load(chunk + nwaves, cur);
+        while (pending >= 16 * TILES) {
+            bool spot[TILES];
+            int qs[TILES];
+            run_tiles<TILES>(ring, head, 16 * TILES, noff, nscl, w1, w2, w3, w4, w5, s_b, n, out,
+                             mode == 1, min_weight, spot, qs);
+            head = (head + 16 * TILES) % QCAP;
+            pending -= 16 * TILES;
+            __builtin_amdgcn_wave_barrier();
+            if (mode == 1)
 #pragma unroll
-        for (int u = 0; u < TILES; u++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][0], h0[u], s_b[(8 + t) * 64 + lane], 0, 0, 0);
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[t][1], h1[u], a[u][t], 0, 0, 0);
-            }
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            h0[u] = act_pair(a[u][0], a[u][1]);
-            h1[u] = act_pair(a[u][2], a[u][3]);
-        }
-#pragma unroll
-        for (int u = 0; u < TILES; u++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][0], h0[u], s_b[(12 + t) * 64 + lane], 0, 0, 0);
-                a[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4[t][1], h1[u], a[u][t], 0, 0, 0);
-            }
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            h0[u] = act_pair(a[u][0], a[u][1]);
-            h1[u] = act_pair(a[u][2], a[u][3]);
-        }
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[0], h0[u], s_b[16 * 64 + lane], 0, 0, 0);
-            y2[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w5[1], h1[u], y2[u], 0, 0, 0);
-        }
-
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            // lanes of group 0 hold outputs 0..3 of their query
-            const int qi = qcur[u];
-            if (g == 0 && qi >= 0) {
-                float v[4];
-#pragma unroll
-                for (int o = 0; o < 4; o++)
-                    v[o] = __builtin_amdgcn_exp2f(y2[u][o]);
-                // spot check: a predicted weight below the threshold has a
-                // large relative error (the log-space fit is absolute); such
-                // queries join the exact closure's list instead
-                if (fbidx && fminf(fminf(v[0], v[1]), v[2]) < min_weight) {
-                    fbidx[atomicAdd(fbcount, 1)] = qi;
-                } else {
-#pragma unroll
-                    for (int o = 0; o < 4; o++)
-                        out[o * n + qi] = v[o];
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < TILES; u++) {
-            qcur[u]  = qnext[u];
-            qnext[u] = qnn[u];
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-                xcur[u][j] = xnext[u][j];
+                for (int u = 0; u < TILES; u++)
+                    fb_append(spot[u], qs[u]);
         }
     }
+    if (pending > 0) {
+        bool spot[TILES];
+        int qs[TILES];
+        run_tiles<TILES>(ring, head, pending, noff, nscl, w1, w2, w3, w4, w5, s_b, n, out,
+                         mode == 1, min_weight, spot, qs);
+        if (mode == 1)
+#pragma unroll
+            for (int u = 0; u < TILES; u++)
+                fb_append(spot[u], qs[u]);
+    }
+    if (mode == 1)
+        fb_flush();
 }
 
 __global__ void k_zero_int(int* p) { *p = 0; }
@@ -730,11 +714,8 @@ struct Context {
     // inference
     bf16x8* frag = nullptr;
     f32x4* biasp = nullptr;
-    unsigned char* cells = nullptr;
-    int* fbn = nullptr;     // network-list count
+    unsigned long long cell_mask = 0;  // routed cells (bit c: cell c)
     float min_weight = 0.0f;  // spot-check threshold on predicted weights
-    int* netlist = nullptr;  // network-list indices (grown on demand)
-    int64_t netcap = 0;
     bool packed = false;
     std::vector<float> host_params;
 };
@@ -765,7 +746,6 @@ int ctx_alloc(Context* c)
         { (void**)&c->bpart, (size_t)nblk * 5 * H * 4 },
         { (void**)&c->lpart, (size_t)nblk * 4 },
         { (void**)&c->frag, (size_t)NFRAG * 64 * 16 }, { (void**)&c->biasp, (size_t)NBIAS * 64 * 16 },
-        { (void**)&c->cells, (size_t)NCELL }, { (void**)&c->fbn, sizeof(int) },
     };
     for (auto& it : items) {
         if (int rc = check_hip(hipMalloc(it.p, it.bytes), "hipMalloc(surrogate)"))
@@ -775,8 +755,6 @@ int ctx_alloc(Context* c)
         return rc;
     if (int rc = check_hip(hipMemset(c->Vv, 0, NPARAM * 4), "hipMemset"))
         return rc;
-    if (int rc = check_hip(hipMemset(c->cells, 0, NCELL), "hipMemset"))
-        return rc;
     return OSLMI_OK;
 }
 
@@ -785,7 +763,7 @@ void ctx_free(Context* c)
     void* ptrs[] = { c->P, c->Mm, c->Vv, c->G, c->Wb, c->WTb, c->X, c->XT, c->Y, c->Z[1], c->Z[2],
                      c->Z[3], c->Z[4], c->Z[5], c->A[1], c->A[2], c->A[3], c->A[4], c->AT[1],
                      c->AT[2], c->AT[3], c->AT[4], c->dZ, c->dZT, c->dA, c->wpart, c->bpart,
-                     c->lpart, c->frag, c->biasp, c->cells, c->fbn, c->netlist };
+                     c->lpart, c->frag, c->biasp };
     for (void* p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1061,8 +1039,12 @@ extern "C" int oslmi_surrogate_set_routing(oslmi_surrogate* s, const unsigned ch
 {
     if (!s || !cells || count != NCELL)
         return set_error(OSLMI_ERR_INVALID, "oslmi_surrogate_set_routing: need 64 cells");
-    return check_hip(hipMemcpy(s->c.cells, cells, NCELL, hipMemcpyHostToDevice),
-                     "hipMemcpy(routing)");
+    unsigned long long m = 0;
+    for (int k = 0; k < NCELL; k++)
+        if (cells[k])
+            m |= 1ull << k;
+    s->c.cell_mask = m;
+    return OSLMI_OK;
 }
 
 extern "C" int oslmi_surrogate_set_spot_check(oslmi_surrogate* s, float min_weight)
@@ -1073,11 +1055,14 @@ extern "C" int oslmi_surrogate_set_spot_check(oslmi_surrogate* s, float min_weig
     return OSLMI_OK;
 }
 
+// Persistent grid of the fused kernel: the workgroups that fit at once (two
+// waves per SIMD: 228 of the 512 registers per lane, so 2 workgroups of 4
+// waves per CU), fewer when there are fewer 64-query chunks than waves.
 static unsigned infer_grid(int64_t n)
 {
-    int64_t waves = (n + 15) / 16;
-    int64_t blocks = (waves + 3) / 4;
-    const int64_t cap = 256 * 8;
+    const int64_t chunks = (n + 63) / 64;
+    int64_t blocks       = (chunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    const int64_t cap    = 256 * 2;
     if (blocks > cap)
         blocks = cap;
     return (unsigned)(blocks < 1 ? 1 : blocks);
@@ -1101,27 +1086,10 @@ extern "C" int oslmi_surrogate_infer(oslmi_surrogate* s, const oslmi_layered_mat
     const int mode = m ? 1 : 0;
     if (mode == 1 && (!d_fallback_index || !d_fallback_count))
         return set_error(OSLMI_ERR_INVALID, "oslmi_surrogate_infer: fallback buffers needed");
-    if (s->c.netcap < n) {
-        if (s->c.netlist) {
-            if (int rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize"))
-                return rc;
-            (void)hipFree(s->c.netlist);
-            s->c.netlist = nullptr;
-        }
-        if (int rc = check_hip(hipMalloc((void**)&s->c.netlist, n * sizeof(int)),
-                               "hipMalloc(network list)"))
-            return rc;
-        s->c.netcap = n;
-    }
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, s->c.fbn);
     if (mode == 1)
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_fallback_count);
-    const int64_t rblocks = (n + ROUTE_CHUNK - 1) / ROUTE_CHUNK;
-    hipLaunchKernelGGL(k_route, dim3((unsigned)rblocks), dim3(256), 0, st, s->c.dom, s->c.cells,
-                       mode, d_query, n, d_out, s->c.netlist, s->c.fbn, d_fallback_index,
-                       d_fallback_count);
-    hipLaunchKernelGGL(k_infer<2>, dim3(infer_grid(n)), dim3(256), 0, st, kc, s->c.frag,
-                       s->c.biasp, s->c.netlist, s->c.fbn, d_query, n, d_out, s->c.min_weight,
+    hipLaunchKernelGGL(k_infer_fused<2>, dim3(infer_grid(n)), dim3(256), 0, st, kc, s->c.cell_mask, mode,
+                       s->c.frag, s->c.biasp, d_query, n, d_out, s->c.min_weight,
                        mode == 1 ? d_fallback_index : nullptr,
                        mode == 1 ? d_fallback_count : nullptr);
     if (int rc = check_launch("oslmi_surrogate_infer"))

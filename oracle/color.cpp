@@ -345,19 +345,8 @@ V3 xyY2xyz(V3 c)
     return V3(Y_y * c.x, c.z, Y_y * (1.0f - c.x - c.y));
 }
 
-// OpenImageIO safe_pow: libm pow with its special cases and clamping
-float spow(float x, float y)
-{
-    if (y == 0.0f)
-        return 1.0f;
-    if (x == 0.0f)
-        return 0.0f;
-    if (x < 0.0f && y != std::floor(y))
-        return 0.0f;
-    float r = std::pow(x, y);
-    const float big = 3.40282346638528859812e+38f;
-    return r < -big ? -big : (r > big ? big : r);
-}
+// OpenImageIO safe_pow, with the oracle's own pow (omath.h)
+float spow(float x, float y) { return osafe_powf(x, y); }
 float s2lin(float x)
 {
     return x <= 0.04045f ? x * (1.0f / 12.92f) : spow((x + 0.055f) * (1.0f / 1.055f), 2.4f);

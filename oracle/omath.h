@@ -47,6 +47,14 @@ inline float ofast_safe_powf(float x, float y)
     return std::pow(x, y);
 }
 inline float ofast_cbrtf(float x) { return std::cbrt(x); }
+inline float osafe_powf(float x, float y)
+{
+    if (y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f || (x < 0.0f && y != std::floor(y)))
+        return 0.0f;
+    return std::pow(x, y);
+}
 #else
 
 inline float bits_f(uint32_t u)
@@ -223,6 +231,23 @@ inline float ofast_safe_powf(float x, float y)
     if (x < 0.0f)
         return 0.0f;
     return ofast_exp2f(ofast_log2f(x) * y);
+}
+
+// x^y with the special cases of OpenImageIO's safe_pow (y == 0 -> 1, x == 0
+// or a negative x with a non-integer y -> 0, the result clamped to the
+// finite range), as 2^(log2|x| * y), negated for a negative x and an odd
+// integer y.
+inline float osafe_powf(float x, float y)
+{
+    if (y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f || (x < 0.0f && y != std::floor(y)))
+        return 0.0f;
+    float r = ofast_exp2f(ofast_log2f(std::fabs(x)) * y);
+    if (x < 0.0f && std::fmod(y, 2.0f) != 0.0f)
+        r = -r;
+    const float big = 3.40282346638528859812e+38f;
+    return r < -big ? -big : (r > big ? big : r);
 }
 
 // Cube root: bit-level initial guess, then three Newton steps

@@ -152,8 +152,11 @@ def test_full_routing_reproduces_exact(dev):
     s.infer(q, out, material=m, fb_index=idx, fb_count=cnt)
     torch.cuda.synchronize()
     assert torch.equal(out, exact)
-    upper = int((q[4] > 0).logical_and(q[2] > 0).sum())
-    assert int(cnt.item()) == upper
+    upper = (q[4] > 0).logical_and(q[2] > 0)
+    assert int(cnt.item()) == int(upper.sum())
+    # the list holds every routed query once (in no particular order)
+    listed = np.sort(idx[:int(cnt.item())].cpu().numpy())
+    assert np.array_equal(listed, np.flatnonzero(upper.cpu().numpy()))
     s.set_routing(np.zeros(S.NCELL, np.uint8))
     s.infer(q, out, material=m, fb_index=idx, fb_count=cnt)
     torch.cuda.synchronize()
