@@ -304,6 +304,24 @@ int oslmi_batch_capture(oslmi_batch* b, void* stream);
 int oslmi_batch_autotune(oslmi_batch* b, void* stream, int reps, int graph,
                          oslmi_batch_config* best, double* best_ms);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): oslmi_batch_bind + oslmi_batch_launch in one call that
+ * allocates nothing. The argument blocks, tile list and tile counters go to
+ * a fixed workspace of the library in each GPU's memory (1 MiB: at most 256
+ * items and 32768 tiles of block * ppt points under the batch's current
+ * launch shape; larger requests fail with OSLMI_ERR_INVALID) instead of
+ * buffers the batch allocates. Needs oslmi_init on the device first (it
+ * sets the workspace up), and the shading system's transform table already
+ * in device memory if a group uses named spaces (one earlier execute or
+ * bind). Runs on `stream` only (no side streams) and returns once its
+ * argument copies have reached the device, which waits for the stream's
+ * earlier work. Every call on a GPU shares that GPU's workspace: calls that
+ * overlap on one GPU (on different streams or threads) race. The batch's
+ * own bound items and captured graph are left untouched. */
+int oslmi_batch_launch_noalloc(oslmi_batch* b, int nitems, const oslmi_batch_item* items,
+                               void* stream);
+
 /* ---- Prebuilt group lowerings -----------------------------------------
  * The two benchmark groups compiled ahead of time (OSL sources in
  * openshadinglanguage_amd/shaders/). Each replaces one

@@ -57,6 +57,7 @@ def _declare(L):
         "oslmi_batch_launch_per_group": ([P, P], I),
         "oslmi_batch_capture": ([P, P], I),
         "oslmi_batch_autotune": ([P, P, I, I, P, P], I),
+        "oslmi_batch_launch_noalloc": ([P, I, P, P], I),
         "oslmi_layered_closure": ([P, I64, P, P, P, P, P, P, P, P], I),
         "oslmi_layered_eval_local": ([P, I64, P, P, P], I),
         "oslmi_bsdl_tables": ([P, I64], I),

@@ -83,6 +83,9 @@ struct Batch {
     std::vector<hipStream_t> side;
     hipEvent_t fork_ev = nullptr;
     std::vector<hipEvent_t> join_ev;
+    // host staging of the no-alloc entry point (argument blocks, tiles)
+    std::vector<GroupArgs> na_args;
+    std::vector<BatchTile> na_tiles;
     // captured launches
     bool in_capture          = false;
     hipGraph_t graph         = nullptr;
@@ -809,6 +812,84 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, int graph,
         *best = win;
     if (best_ms)
         *best_ms = win_ms;
+    return OSLMI_OK;
+}
+
+static_assert(sizeof(GroupArgs) <= 640, "no-alloc workspace: argument block slot too small");
+static_assert(kNoAllocArgsOffset >= kNoAllocMaxCounters * sizeof(int32_t), "workspace layout");
+static_assert(kNoAllocTilesOffset + kNoAllocMaxTiles * sizeof(BatchTile) <= kNoAllocScratchBytes,
+              "workspace layout");
+
+int oslmi_batch_launch_noalloc(oslmi_batch* h, int nitems, const oslmi_batch_item* items,
+                               void* stream)
+{
+    if (!h || (nitems > 0 && !items))
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch_noalloc: null argument");
+    Batch& b       = h->b;
+    hipStream_t st = (hipStream_t)stream;
+    if (nitems != (int)b.groups.size())
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch_noalloc: nitems must equal the batch's group count");
+    if (b.parts.empty() || !b.parts[0].fn)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch_noalloc: batch kernels not loaded");
+    const int ncount = nitems + (int)b.parts.size();
+    if (nitems > kNoAllocMaxItems || ncount > kNoAllocMaxCounters)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_batch_launch_noalloc: more than "
+                                                + std::to_string(kNoAllocMaxItems) + " items");
+    unsigned char* base = nullptr;
+    if (int rc = noalloc_scratch(&base))
+        return rc;
+    // argument blocks (tables must already be resident: nothing is allocated)
+    std::vector<int64_t> counts(nitems);
+    b.na_args.resize(nitems);
+    for (int k = 0; k < nitems; ++k) {
+        const oslmi_batch_item& it = items[k];
+        std::string who = "oslmi_batch_launch_noalloc item " + std::to_string(k);
+        if (int rc = make_group_args(*b.ss, *b.groups[k], &it.globals, it.d_outputs, it.d_wo,
+                                     it.d_wi, it.d_eval, who.c_str(), b.na_args[k], false))
+            return rc;
+        if (it.globals.n > ((int64_t)1 << 40))
+            return set_error(OSLMI_ERR_INVALID, who + ": n too large");
+        counts[k] = it.globals.n;
+    }
+    const float* bbp = nullptr;
+    if (b.any_bb) {
+        const ColorSysDev* cs = nullptr;
+        if (int rc = device_colorsys(&cs, false))
+            return rc;
+        bbp = (const float*)((const char*)cs + offsetof(ColorSysDev, bb));
+    }
+    std::vector<int> first, count, part_first, part_ntiles;
+    if (int rc = b.tile_list(counts, b.na_tiles, first, count, part_first, part_ntiles))
+        return rc;
+    if ((int64_t)b.na_tiles.size() > kNoAllocMaxTiles)
+        return set_error(OSLMI_ERR_INVALID,
+                         "oslmi_batch_launch_noalloc: " + std::to_string(b.na_tiles.size())
+                             + " tiles exceed the workspace's " + std::to_string(kNoAllocMaxTiles)
+                             + " (larger block or ppt, or fewer points per call)");
+    int32_t* d_counter   = (int32_t*)base;
+    GroupArgs* d_args    = (GroupArgs*)(base + kNoAllocArgsOffset);
+    BatchTile* d_tiles   = (BatchTile*)(base + kNoAllocTilesOffset);
+    if (nitems > 0)
+        if (int rc = check_hip(hipMemcpyAsync(d_args, b.na_args.data(), nitems * sizeof(GroupArgs),
+                                              hipMemcpyHostToDevice, st),
+                               "hipMemcpyAsync(no-alloc args)"))
+            return rc;
+    if (!b.na_tiles.empty())
+        if (int rc = check_hip(hipMemcpyAsync(d_tiles, b.na_tiles.data(),
+                                              b.na_tiles.size() * sizeof(BatchTile),
+                                              hipMemcpyHostToDevice, st),
+                               "hipMemcpyAsync(no-alloc tiles)"))
+            return rc;
+    // counters start at zero whatever an earlier call left
+    if (int rc = noalloc_clear(d_counter, ncount, st))
+        return rc;
+    // the host staging buffers are reused by the next call
+    if (int rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize"))
+        return rc;
+    for (size_t q = 0; q < b.parts.size(); ++q)
+        if (int rc = b.launch_range_on(b.parts[q], part_first[q], part_ntiles[q], d_args, d_tiles,
+                                       d_counter + nitems + (int)q, bbp, st))
+            return rc;
     return OSLMI_OK;
 }
 

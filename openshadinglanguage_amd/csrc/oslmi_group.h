@@ -162,15 +162,34 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
 int build_code_object(const ShadingSys& ss, const std::string& src, const std::string& name,
                       std::vector<char>& code, double* seconds);
 
-// The colour system (matrices, blackbody table) in device memory.
+// The colour system (matrices, blackbody table) in device memory, built on
+// first use of a device; with allocate false it only reports an existing
+// copy (an error if this device has none yet).
 struct ColorSysDev;
-int device_colorsys(const ColorSysDev** out);
+int device_colorsys(const ColorSysDev** out, bool allocate = true);
 
 // Kernel argument block of one execution of group g (validates the
 // arguments like BatchedExecutor::execute; `who` prefixes error messages).
+// With allocate false, tables the group reads (colour system, transform
+// table) must already be in device memory.
 int make_group_args(const ShadingSys& ss, const Group& g, const oslmi_batched_globals* bg,
                     float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
-                    const char* who, GroupArgs& A);
+                    const char* who, GroupArgs& A, bool allocate = true);
+
+// The no-alloc workspace (oslmi_noalloc.hip): one fixed device array per
+// GPU, laid out as tile counters, argument blocks, tiles.
+constexpr int kNoAllocMaxCounters   = 256;
+constexpr int kNoAllocMaxItems      = 256;
+constexpr int kNoAllocMaxTiles      = 32768;
+constexpr size_t kNoAllocArgsOffset = 1024;
+constexpr size_t kNoAllocTilesOffset = kNoAllocArgsOffset + 256 * 640;
+constexpr size_t kNoAllocScratchBytes = 1 << 20;
+// looks up this device's workspace address (with the device current)
+int noalloc_scratch_init();
+// the workspace of the current device (oslmi_init must have run on it)
+int noalloc_scratch(unsigned char** out);
+// zero-fills n tile counters on the stream
+int noalloc_clear(int32_t* p, int n, hipStream_t stream);
 }  // namespace oslmi
 
 // Opaque handles of the C ABI.

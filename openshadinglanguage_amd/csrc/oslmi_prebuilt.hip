@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_layered3(const float* __restrict__ P,
 static std::mutex g_cs_mutex;
 static std::vector<ColorSysDev*> g_cs_dev;
 
-int device_colorsys(const ColorSysDev** out)
+int device_colorsys(const ColorSysDev** out, bool allocate)
 {
     int dev = 0;
     if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice"))
@@ -166,6 +166,9 @@ int device_colorsys(const ColorSysDev** out)
     std::lock_guard<std::mutex> lock(g_cs_mutex);
     if ((int)g_cs_dev.size() <= dev)
         g_cs_dev.resize(dev + 1, nullptr);
+    if (!g_cs_dev[dev] && !allocate)
+        return set_error(OSLMI_ERR_INVALID,
+                         "colour system not yet in this device's memory: call oslmi_init first");
     if (!g_cs_dev[dev]) {
         ColorSysDev host;
         if (!build_colorsys("Rec709", host))
@@ -210,7 +213,7 @@ extern "C" int oslmi_shade_layered3(int64_t n, const float* d_P, const float* d_
     if (!d_P || !d_N || !d_wo || !d_wi || !dPdx || !dPdy || !d_out)
         return set_error(OSLMI_ERR_INVALID, "oslmi_shade_layered3: null argument");
     const ColorSysDev* cs = nullptr;
-    if (int rc = device_colorsys(&cs))
+    if (int rc = device_colorsys(&cs, true))
         return rc;
     Layered3Uniform U;
     for (int c = 0; c < 3; ++c) {

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "oslmi.h"
+#include "oslmi_group.h"
 #include "oslmi_internal.h"
 
 namespace oslmi {
@@ -55,5 +56,10 @@ extern "C" int oslmi_init(void)
     std::string arch(prop.gcnArchName);
     if (arch.rfind("gfx950", 0) != 0)
         return set_error(OSLMI_ERR_UNSUPPORTED, "oslmi_init: device is " + arch + ", library is built for gfx950");
-    return OSLMI_OK;
+    // per-device set-up: the colour system's tables in device memory and
+    // the address of the no-alloc workspace on this device
+    const ColorSysDev* cs = nullptr;
+    if (int rc = device_colorsys(&cs))
+        return rc;
+    return noalloc_scratch_init();
 }

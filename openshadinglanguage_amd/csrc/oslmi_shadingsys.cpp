@@ -31,7 +31,6 @@
 
 namespace oslmi {
 
-int device_colorsys(const ColorSysDev** out);  // oslmi_prebuilt.hip
 
 namespace {
 
@@ -150,7 +149,7 @@ int build_code_object(const ShadingSys& ss, const std::string& src, const std::s
 
 int make_group_args(const ShadingSys& ss, const Group& g, const oslmi_batched_globals* bg,
                     float* d_outputs, const float* d_wo, const float* d_wi, float* d_eval,
-                    const char* who, GroupArgs& A)
+                    const char* who, GroupArgs& A, bool allocate)
 {
     std::string w(who);
     if (bg->n < 0)
@@ -174,11 +173,16 @@ int make_group_args(const ShadingSys& ss, const Group& g, const oslmi_batched_gl
     A.eval = d_eval;
     if (g.uses_colorsys) {
         const ColorSysDev* cs = nullptr;
-        if (int rc = device_colorsys(&cs))
+        if (int rc = device_colorsys(&cs, allocate))
             return rc;
         A.colorsys = cs;
     }
     if (g.uses_xforms) {
+        if (!allocate && !ss.xf.device)
+            return set_error(OSLMI_ERR_INVALID,
+                             w + ": the transform table is not yet in device memory (run one "
+                                 "oslmi_execute_batch or oslmi_batch_bind of the shading system "
+                                 "first)");
         if (int rc = xform_table_sync(ss))
             return rc;
         A.xf = ss.xf.device;
