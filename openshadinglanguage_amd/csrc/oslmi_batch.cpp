@@ -815,7 +815,7 @@ int oslmi_batch_autotune(oslmi_batch* h, void* stream, int reps, int graph,
     return OSLMI_OK;
 }
 
-static_assert(sizeof(GroupArgs) <= 640, "no-alloc workspace: argument block slot too small");
+static_assert(sizeof(GroupArgs) <= kNoAllocArgsSlot, "no-alloc workspace: argument block slot too small");
 static_assert(kNoAllocArgsOffset >= kNoAllocMaxCounters * sizeof(int32_t), "workspace layout");
 static_assert(kNoAllocTilesOffset + kNoAllocMaxTiles * sizeof(BatchTile) <= kNoAllocScratchBytes,
               "workspace layout");

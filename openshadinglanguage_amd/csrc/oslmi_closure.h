@@ -21,14 +21,26 @@
 //     sample_cosine_hemisphere, to_unit_disk  sampling.h:70-100
 //   Microfacet<GGXDist,0>::sample  src/testrender/shading.cpp:655-685,
 //     sampleMicronormal :759-795, GGXDist::sampleSlope :483-507
+//
+// Kernels whose closures include MaterialX lobes (dielectric_bsdf,
+// generalized_schlick_bsdf, sheen_bsdf, layer) are compiled with
+// OSLMI_MX_CLOSURES defined: their lobes then also carry a libbsdl lobe
+// state (oslmi_mtx.h), and closure values the layer opacity
+// (testrender's evaluate_layer_opacity) of their last component.
 #pragma once
 #include "oslmi_math.h"
 #include "oslmi_types.h"
+#ifdef OSLMI_MX_CLOSURES
+#include "oslmi_mtx.h"
+#endif
 
 namespace oslmi {
 
-// Closure ids as registered by testrender (shading.cpp:185-296 order).
-enum ClosureId { kClosureDiffuse = 1, kClosureMicrofacetGGX = 2, kClosureMicrofacetBeckmann = 3 };
+// Lobe kinds (the library's own numbering).
+enum ClosureId {
+    kClosureDiffuse = 1, kClosureMicrofacetGGX = 2, kClosureMicrofacetBeckmann = 3,
+    kClosureMxDielectric = 4, kClosureMxSchlick = 5, kClosureMxSheen = 6
+};
 
 struct Lobe {
     int id;
@@ -36,6 +48,9 @@ struct Lobe {
     V3 N;
     float alpha;
     float eta;
+#ifdef OSLMI_MX_CLOSURES
+    mtx::MxLobe mx;
+#endif
 };
 
 // Every access below uses a compile-time slot index (loops are unrolled
@@ -45,7 +60,19 @@ template<int MAXL> struct LobeList {
     int n;
     Lobe l[MAXL];
     float pdf[MAXL];
+#ifdef OSLMI_MX_CLOSURES
+    // nn: the value is not the null closure (a layer of two null closures
+    // is not null yet holds no lobe); op: evaluate_layer_opacity of it.
+    // testrender's closure walks stop at the first null pointer they pop
+    // (a layer pushes its base even when that is null): tr / opend, the
+    // walk of process_bsdf_closure / evaluate_layer_opacity ends inside
+    // this value, so nothing queued before it is visited.
+    int nn, tr, opend;
+    V3 op;
+    OSLMI_DEV LobeList() : n(0), nn(0), tr(0), opend(0), op(0.0f) {}
+#else
     OSLMI_DEV LobeList() : n(0) {}
+#endif
     // copy of a list of another capacity (lobes beyond MAXL are dropped,
     // as add() drops them)
     template<int K2> OSLMI_DEV LobeList(const LobeList<K2>& o) : n(0)
@@ -55,6 +82,12 @@ template<int MAXL> struct LobeList {
             if (k < o.n)
                 l[k] = o.l[k];
         n = o.n < MAXL ? o.n : MAXL;
+#ifdef OSLMI_MX_CLOSURES
+        nn    = o.nn;
+        tr    = o.tr;
+        opend = o.opend;
+        op    = o.op;
+#endif
     }
     OSLMI_DEV void add(const Lobe& x)
     {
@@ -158,6 +191,16 @@ OSLMI_DEV float beckmann_D(V3 Hr, float xa, float ya)
 OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
 {
     BsdfEval r;
+#ifdef OSLMI_MX_CLOSURES
+    // MaterialX lobes: eval_impl in the lobe frame at the wo they were
+    // built at (BSDL_WRAP::eval, src/testrender/shading.cpp:88-93)
+    if (l.id >= kClosureMxDielectric) {
+        const mtx::Sample s = mtx::mx_eval_local(l.mx, l.id - kClosureMxDielectric, l.mx.f.local(wi));
+        r.weight            = s.weight;
+        r.pdf               = s.pdf;
+        return r;
+    }
+#endif
     if (l.id == kClosureDiffuse) {
         r.weight = V3(1.0f);
         r.pdf    = fmaxf(dot(l.N, wi), 0.0f) * 0.31830987334251403809f;  // float(M_1_PI)
@@ -188,6 +231,10 @@ OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
 
 OSLMI_DEV V3 lobe_albedo(const Lobe& l, V3 wo)
 {
+#ifdef OSLMI_MX_CLOSURES
+    if (l.id >= kClosureMxDielectric)
+        return mtx::mx_albedo(l.mx);
+#endif
     if (l.id == kClosureDiffuse)
         return V3(1.0f);
     return V3(fresnel_dielectric(dot(l.N, wo), l.eta));
@@ -309,6 +356,21 @@ OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
     r.wi     = V3(0.0f);
     r.weight = V3(0.0f);
     r.pdf    = 0.0f;
+#ifdef OSLMI_MX_CLOSURES
+    // MaterialX lobes: sample_impl's direction, weighted by eval_impl
+    // there, back to world space (BSDL_WRAP::sample, shading.cpp:94-101)
+    if (l.id >= kClosureMxDielectric) {
+        const int kind = l.id - kClosureMxDielectric;
+        V3 wl;
+        if (!mtx::mx_sample_dir(l.mx, kind, rx, ry, &wl))
+            return r;
+        const mtx::Sample s = mtx::mx_eval_local(l.mx, kind, wl);
+        r.wi                = l.mx.f.world(wl);
+        r.weight            = s.weight;
+        r.pdf               = s.pdf;
+        return r;
+    }
+#endif
     if (l.id == kClosureDiffuse) {
         to_unit_disk(rx, ry);
         const float cos_theta = sqrtf(fmaxf(1 - rx * rx - ry * ry, 0.0f));

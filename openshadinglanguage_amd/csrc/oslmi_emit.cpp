@@ -74,6 +74,7 @@ struct Emitter {
     int cur_layer     = 0;
     std::set<int> globals;
     bool uses_cs = false, uses_bb = false;
+    bool uses_mx = false;  // MaterialX closures (libbsdl lobes, energy tables)
 
     Emitter(const ShadingSys& s, Group& gr) : ss(s), g(gr) {}
 
@@ -318,13 +319,88 @@ struct Emitter {
             ++i;
             if (op.args.empty() || !is_closure(L, op.args[0]))
                 continue;
-            int v = op.name == "closure" ? 1 : 0;
-            if (op.name != "closure")
-                for (size_t k = 1; k < op.args.size(); ++k)
-                    if (is_closure(L, op.args[k]))
-                        v += cap[op.args[k]];
+            // a closure constructor makes one lobe (layer: the lobes of its
+            // two closure arguments); add, mul and assign carry their
+            // closure operands' lobes
+            int v = 0;
+            for (size_t k = 1; k < op.args.size(); ++k)
+                if (is_closure(L, op.args[k]))
+                    v += cap[op.args[k]];
+            if (op.name == "closure" && v == 0)
+                v = 1;
             int& r = cap[op.args[0]];
             r      = std::max(r, std::min(v, cap_max));
+        }
+    }
+
+    // MaterialX closures (stdosl.h:522-707) as libbsdl lobes built at the
+    // direction the closure is evaluated from (mxwo_, the renderer's -I),
+    // and layer(top, base) with testrender's weighting. Positional
+    // arguments as declared; keyword arguments after them in (name, value)
+    // pairs. Reflection only: a transmission tint or an absorption must be
+    // a constant zero, the sheen mode 0 (Conty-Kulla); the distribution
+    // string, thin film, dispersion (no hero wavelength) and labels do not
+    // change the lobes.
+    void mx_closure(int L, const OOp& op, const std::string& cname, const std::string& R)
+    {
+        auto a = [&](size_t k) { return op.args[k]; };
+        auto is_zero_const = [&](int sym) {
+            std::vector<float> v;
+            if (S(L, sym).kind != kConst || !numval(L, sym, v))
+                return false;
+            return std::all_of(v.begin(), v.end(), [](float x) { return x == 0.0f; });
+        };
+        const size_t npos = cname == "dielectric_bsdf" ? 8 : cname == "generalized_schlick_bsdf" ? 10
+                          : cname == "sheen_bsdf" ? 3 : 2;
+        if (op.args.size() < 2 + npos) {
+            fail(cname + " needs " + std::to_string(npos) + " arguments");
+            return;
+        }
+        for (size_t k = 2 + npos; k < op.args.size(); k += 2) {
+            std::string key;
+            if (k + 1 >= op.args.size() || !strval(L, a(k), key)) {
+                fail(cname + ": malformed keyword arguments");
+                return;
+            }
+            const bool ignored = key == "label" || key == "thinfilm_thickness" || key == "thinfilm_ior"
+                                 || (key == "dispersion" && cname == "dielectric_bsdf");
+            const bool zero_only = (key == "absorption" && cname == "dielectric_bsdf")
+                                   || (key == "mode" && cname == "sheen_bsdf");
+            if (zero_only && !is_zero_const(a(k + 1))) {
+                fail(cname + ": '" + key + "' other than a constant 0 is not supported");
+                return;
+            }
+            if (!ignored && !zero_only) {
+                fail(cname + ": keyword '" + key + "' is not supported");
+                return;
+            }
+        }
+        if (cname == "layer") {
+            if (!is_closure(L, a(2)) || !is_closure(L, a(3))) {
+                fail("layer needs two closures");
+                return;
+            }
+            uses_mx = true;
+            line(R + " = layer(" + var(L, a(2)) + ", " + var(L, a(3)) + ");");
+            return;
+        }
+        if (cname != "sheen_bsdf" && !is_zero_const(a(5))) {
+            fail(cname + ": transmission (a transmission tint other than a constant 0) is not supported");
+            return;
+        }
+        uses_mx = true;
+        if (cname == "dielectric_bsdf") {
+            line(R + " = closure_mx_dielectric(" + ex(L, a(2), K_V3) + ", " + ex(L, a(3), K_V3) + ", "
+                 + ex(L, a(4), K_V3) + ", " + ex(L, a(6), K_FLOAT) + ", " + ex(L, a(7), K_FLOAT) + ", "
+                 + ex(L, a(8), K_FLOAT) + ", mxwo_, A.bsdl);");
+        } else if (cname == "generalized_schlick_bsdf") {
+            line(R + " = closure_mx_schlick(" + ex(L, a(2), K_V3) + ", " + ex(L, a(3), K_V3) + ", "
+                 + ex(L, a(4), K_V3) + ", " + ex(L, a(6), K_FLOAT) + ", " + ex(L, a(7), K_FLOAT) + ", "
+                 + ex(L, a(8), K_V3) + ", " + ex(L, a(9), K_V3) + ", " + ex(L, a(10), K_FLOAT)
+                 + ", mxwo_, A.bsdl);");
+        } else {
+            line(R + " = closure_mx_sheen(" + ex(L, a(2), K_V3) + ", " + ex(L, a(3), K_V3) + ", "
+                 + ex(L, a(4), K_FLOAT) + ", mxwo_);");
         }
     }
 
@@ -1140,6 +1216,9 @@ struct Emitter {
                 }
                 line(R + " = closure_microfacet_" + dist + "(" + ex(L, a(3), K_V3) + ", " + ex(L, a(5), K_FLOAT)
                      + ", " + ex(L, a(7), K_FLOAT) + ");");
+            } else if (cname == "dielectric_bsdf" || cname == "generalized_schlick_bsdf"
+                       || cname == "sheen_bsdf" || cname == "layer") {
+                mx_closure(L, op, cname, R);
             } else {
                 fail("closure '" + cname + "' is not supported");
             }
@@ -1364,6 +1443,13 @@ struct Emitter {
         std::ostringstream k;
         if (g.surface_closure)
             k << "        Closure ci_;\n";
+        if (uses_mx) {
+            // MaterialX lobes are built at the direction they are evaluated
+            // from: wo when the closure is evaluated or sampled, else -I
+            globals.insert(kG_I);
+            k << "        const V3 mxwo_ = A.wo ? V3(A.wo[i], A.wo[A.n + i], A.wo[2 * A.n + i])\n"
+                 "                              : V3(0.0f) - ld3(A, kG_I, i);\n";
+        }
         k << carriers.str();
         k << b;
         if (g.surface_closure && !ss.closure_sample) {
@@ -1399,6 +1485,7 @@ struct Emitter {
         g.can_sample    = g.surface_closure && ss.closure_sample;
         code            = k.str();
         g.uses_colorsys = uses_cs;
+        g.uses_mx       = uses_mx;
         g.globals_used.assign(globals.begin(), globals.end());
         return true;
     }
@@ -1411,7 +1498,7 @@ struct Emitter {
             return false;
         std::ostringstream k;
         k << "// Emitted by the oslmi group emitter for group '" << g.name << "'.\n";
-        k << prelude();
+        k << prelude(uses_mx);
         k << "extern \"C\" __global__ " << kernel_attributes(ss) << " void oslmi_group_main(GroupArgs A)\n{\n";
         k << "    using Closure = LobeList<" << closure_capacity() << ">;\n";
         if (uses_cs)
@@ -1429,9 +1516,10 @@ struct Emitter {
         return true;
     }
 
-    static std::string prelude()
+    static std::string prelude(bool mx)
     {
-        return "#include \"oslmi_rt.h\"\n#include \"oslmi_groupargs.h\"\n"
+        return std::string(mx ? "#define OSLMI_MX_CLOSURES 1\n" : "")
+               + "#include \"oslmi_rt.h\"\n#include \"oslmi_groupargs.h\"\n"
                "using namespace oslmi;\nusing namespace oslmi::rt;\n"
                "__device__ __forceinline__ V3 ld3(const GroupArgs& A, int g, int64_t i) {\n"
                "    const float* p = A.g[g];\n"
@@ -1470,8 +1558,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
     std::ostringstream k;
     k << "// Emitted by the oslmi group emitter: persistent batch kernel over " << groups.size()
       << " groups.\n";
-    k << Emitter::prelude();
-    bool any_bb = false;
+    bool any_bb = false, any_mx = false;
     std::vector<std::string> cases;
     for (size_t j = 0; j < groups.size(); ++j) {
         const int gi = ids[j];
@@ -1483,6 +1570,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
             return false;
         }
         any_bb = any_bb || e.uses_bb;
+        any_mx = any_mx || e.uses_mx;
         // a case keeps taking tiles while they belong to its group, so the
         // group's loop invariants live only inside its case
         std::ostringstream c;
@@ -1505,6 +1593,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
         c << "        }\n        break;\n    }\n";
         cases.push_back(c.str());
     }
+    k << Emitter::prelude(any_mx);
     k << "// next tile index for the whole workgroup (the barrier in front keeps\n"
          "// the previous index readable until every work-item has taken it)\n"
          "__device__ __forceinline__ int fetch_tile(int32_t* counter, int* s_tile) {\n"

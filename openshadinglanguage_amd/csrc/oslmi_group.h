@@ -69,6 +69,7 @@ struct Group {
     int output_planes        = 0;
     bool surface_closure     = false;  // some layer has the Ci global (shared by all layers)
     bool uses_colorsys       = false;
+    bool uses_mx             = false;  // MaterialX closures: reads the libbsdl energy tables
     bool uses_xforms         = false;  // reads the transform / attribute table
     bool can_sample          = false;  // kernel samples Ci when A.samp is set
     std::vector<unsigned> globals_used;  // GlobalIndex values read
@@ -167,6 +168,11 @@ int build_code_object(const ShadingSys& ss, const std::string& src, const std::s
 // copy (an error if this device has none yet).
 struct ColorSysDev;
 int device_colorsys(const ColorSysDev** out, bool allocate = true);
+// libbsdl's energy tables in device memory (oslmi_layered.hip), uploaded on
+// first use of a device unless allocate is false.
+namespace mtx {
+int device_tables(const float** out, bool allocate = true);
+}
 
 // Kernel argument block of one execution of group g (validates the
 // arguments like BatchedExecutor::execute; `who` prefixes error messages).
@@ -182,7 +188,8 @@ constexpr int kNoAllocMaxCounters   = 256;
 constexpr int kNoAllocMaxItems      = 256;
 constexpr int kNoAllocMaxTiles      = 32768;
 constexpr size_t kNoAllocArgsOffset = 1024;
-constexpr size_t kNoAllocTilesOffset = kNoAllocArgsOffset + 256 * 640;
+constexpr size_t kNoAllocArgsSlot    = 768;  // bytes per argument block
+constexpr size_t kNoAllocTilesOffset = kNoAllocArgsOffset + kNoAllocMaxItems * kNoAllocArgsSlot;
 constexpr size_t kNoAllocScratchBytes = 1 << 20;
 // looks up this device's workspace address (with the device current)
 int noalloc_scratch_init();

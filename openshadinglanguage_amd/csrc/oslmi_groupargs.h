@@ -34,6 +34,8 @@ struct GroupArgs {
     // planes (rx, ry, rz), samp 7 planes (wi xyz, weight rgb, pdf)
     const float* rnd;
     float* samp;
+    // libbsdl energy tables (kernels with MaterialX closures)
+    const float* bsdl;
 };
 
 // One unit of work of the persistent batch kernel: points [begin, end) of

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "oslmi.h"
+#include "oslmi_group.h"
 #include "oslmi_internal.h"
 #include "oslmi_mtx.h"
 
@@ -230,16 +231,20 @@ const float* host_tables()
     return g_host_tables.data();
 }
 
-int device_tables(const float** out)
+int device_tables(const float** out, bool allocate)
 {
     int dev = 0;
     if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice"))
         return rc;
-    const float* host = host_tables();
     std::lock_guard<std::mutex> lock(g_dev_mutex);
     if ((int)g_dev_tables.size() <= dev)
         g_dev_tables.resize(dev + 1, nullptr);
+    if (!g_dev_tables[dev] && !allocate)
+        return set_error(OSLMI_ERR_INVALID,
+                         "libbsdl energy tables not yet in this device's memory (run one "
+                         "allocating execution of a MaterialX closure group first)");
     if (!g_dev_tables[dev]) {
+        const float* host = host_tables();
         float* d = nullptr;
         if (int rc = check_hip(hipMalloc((void**)&d, kLutFloats * sizeof(float)),
                                "hipMalloc(bsdl tables)"))
@@ -279,47 +284,6 @@ struct Layered {
     float pdf[3];
 };
 
-OSLMI_DEV float conty_kulla_albedo(float a, float cosNO)
-{
-    float rx = 13.67300f, ry = 1.0f;
-    rx = rx + -68.78018f * cosNO;
-    ry = ry + 61.57746f * cosNO;
-    rx = rx + 799.08825f * a;
-    ry = ry + 442.78211f * a;
-    rx = rx + -905.00061f * cosNO * a;
-    ry = ry + 2597.49308f * cosNO * a;
-    rx = rx + 60.28956f * cosNO * cosNO;
-    ry = ry + 121.81241f * cosNO * cosNO;
-    rx = rx + 1086.96473f * a * a;
-    ry = ry + 3045.55075f * a * a;
-    return CLAMP(rx / ry, 0.0f, 1.0f);
-}
-OSLMI_DEV float conty_kulla_D(float a, V3 Hr)
-{
-    float cos_theta = CLAMP(Hr.z, 0.0f, 1.0f);
-    float sin_theta = sqrtf(1.0f - SQR(cos_theta));
-    return fsafe_pow(sin_theta, 1 / a) * (2 + 1 / a) * 0.5f * kONEOVERPI;
-}
-// SheenMicrofacet<ContyKullaDist<false>>::eval, weight times the tint
-OSLMI_DEV Sample sheen_mf_eval(float a, V3 tint, V3 wo, V3 wi)
-{
-    const float cosNO = wo.z;
-    const float cosNI = wi.z;
-    if (cosNI <= 1e-5f || cosNO <= 1e-5f)
-        return zero_sample();
-    const float D = conty_kulla_D(a, normalized(wo + wi));
-    if ((double)D < 1e-6)
-        return zero_sample();
-    const float ni = smin(1.0f, wi.z);
-    const float no = smin(1.0f, wo.z);
-    const float G2 = (ni * no) / (ni + no - ni * no);
-    Sample s;
-    s.wi        = wi;
-    s.weight    = V3(D * G2 * 0.5f * kPI / cosNO) * tint;
-    s.pdf       = 0.5f * kONEOVERPI;
-    s.roughness = 0.0f;
-    return s;
-}
 // SheenLobe::eval_impl (front facing)
 OSLMI_DEV Sample sheen_eval(const Layered& L, V3 wo, V3 wi)
 {

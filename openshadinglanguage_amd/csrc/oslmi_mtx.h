@@ -433,5 +433,157 @@ template<class S> OSLMI_HD Sample eval_turquin(const S& sp, V3 wo, V3 wi)
     return s;
 }
 
+// ---- sheen (Conty-Kulla) --------------------------------------------------
+OSLMI_DEV float conty_kulla_albedo(float a, float cosNO)
+{
+    float rx = 13.67300f, ry = 1.0f;
+    rx = rx + -68.78018f * cosNO;
+    ry = ry + 61.57746f * cosNO;
+    rx = rx + 799.08825f * a;
+    ry = ry + 442.78211f * a;
+    rx = rx + -905.00061f * cosNO * a;
+    ry = ry + 2597.49308f * cosNO * a;
+    rx = rx + 60.28956f * cosNO * cosNO;
+    ry = ry + 121.81241f * cosNO * cosNO;
+    rx = rx + 1086.96473f * a * a;
+    ry = ry + 3045.55075f * a * a;
+    return CLAMP(rx / ry, 0.0f, 1.0f);
+}
+OSLMI_DEV float conty_kulla_D(float a, V3 Hr)
+{
+    float cos_theta = CLAMP(Hr.z, 0.0f, 1.0f);
+    float sin_theta = sqrtf(1.0f - SQR(cos_theta));
+    return fsafe_pow(sin_theta, 1 / a) * (2 + 1 / a) * 0.5f * kONEOVERPI;
+}
+// SheenMicrofacet<ContyKullaDist<false>>::eval, weight times the tint
+OSLMI_DEV Sample sheen_mf_eval(float a, V3 tint, V3 wo, V3 wi)
+{
+    const float cosNO = wo.z;
+    const float cosNI = wi.z;
+    if (cosNI <= 1e-5f || cosNO <= 1e-5f)
+        return zero_sample();
+    const float D = conty_kulla_D(a, normalized(wo + wi));
+    if ((double)D < 1e-6)
+        return zero_sample();
+    const float ni = smin(1.0f, wi.z);
+    const float no = smin(1.0f, wo.z);
+    const float G2 = (ni * no) / (ni + no - ni * no);
+    Sample s;
+    s.wi        = wi;
+    s.weight    = V3(D * G2 * 0.5f * kPI / cosNO) * tint;
+    s.pdf       = 0.5f * kONEOVERPI;
+    s.roughness = 0.0f;
+    return s;
+}
+
+// ---- one MaterialX lobe of a closure list --------------------------------
+// A libbsdl MTX lobe as testrender's BSDL_WRAP builds it at wo (the
+// renderer's -I; src/testrender/shading.cpp:73-106): the lobe frame
+// (visible normal, tangent U; the sheen's X is wo), wo in that frame and
+// the state its eval, sample, albedo and layer opacity read. Front facing,
+// path roughness 0, reflection only.
+enum MxKind { kMxDielectric = 0, kMxSchlick = 1, kMxSheen = 2 };
+struct MxLobe {
+    Frame f;
+    V3 wol;     // wo in f
+    V3 tint;    // reflection tint; sheen albedo
+    Spec sp;    // dielectric / generalized Schlick specular
+    float E;    // dielectric / Schlick E_ms (0 without reflection); sheen Emiss
+    float a;    // sheen ContyKulla alpha
+    int dorefl;
+};
+
+// DielectricLobe (kind 0) / SchlickLobe (kind 1) constructors
+// (MTX/bsdf_dielectric_impl.h:210-262, MTX/bsdf_schlick_impl.h:14-60):
+// anisotropic GGX from roughness_x / roughness_y, Fresnel, the
+// MiniMicrofacetGGX and DielectricReflFront energies at cos(N, wo).
+OSLMI_DEV void mx_micro(MxLobe* M, int kind, V3 N, V3 U, V3 refl_tint, float roughness_x,
+                        float roughness_y, float ior, V3 F0, V3 F90, float exponent, V3 wo,
+                        const float* T)
+{
+    M->f      = frame_zx(visible_normal(wo, N, N), U);
+    M->wol    = M->f.local(wo);
+    M->tint   = refl_tint;
+    M->dorefl = MAX_RGB(refl_tint) > 0;
+    M->a      = 0.0f;
+    const float rx    = CLAMP(roughness_x, kEPSILON, 2.0f);
+    const float ry    = CLAMP(roughness_y, kEPSILON, 2.0f);
+    const float ax    = smax(rx, ry);
+    const float ay    = smin(rx, ry);
+    const float b     = ay / ax;
+    const float aniso = (1 - b) / (1 + b);
+    const float rough = regularize_roughness(sqrtf(ax / (1 + aniso)));
+    const float cosNO = vdot(M->f.Z, wo);
+    if (kind == kMxDielectric) {
+        M->sp.kind     = 0;
+        M->sp.eta      = dfresnel_eta(CLAMP(ior, kIOR_MIN, kIOR_MAX) / 1.0f, false);
+        M->sp.F0       = V3(0.0f);
+        M->sp.F90      = V3(0.0f);
+        M->sp.exponent = 0.0f;
+        M->sp.tir_cos  = 0.0f;
+    } else {
+        const float avg_F0 = CLAMP((F0.x + F0.y + F0.z) * (1.0f / 3), 0.0f, 0.99f);
+        const float sqF0   = sqrtf(avg_F0);
+        M->sp.kind         = 1;
+        M->sp.eta          = dfresnel_eta((1 + sqF0) / (1 - sqF0), false);
+        M->sp.F0 = V3(smax(0.0f, smin(1.0f, F0.x)), smax(0.0f, smin(1.0f, F0.y)), smax(0.0f, smin(1.0f, F0.z)));
+        M->sp.F90 = V3(smax(0.0f, smin(1.0f, F90.x)), smax(0.0f, smin(1.0f, F90.y)),
+                       smax(0.0f, smin(1.0f, F90.z)));
+        M->sp.exponent = exponent;
+        M->sp.tir_cos  = M->sp.eta >= 1 ? 0 : sqrtf(1 - SQR(M->sp.eta));
+    }
+    M->sp.d    = ggx_make(rough, aniso, rx < ry);
+    M->sp.E_ms = emiss_eval(T, kLutMiniGGX, rough, 0.0f, cosNO);
+    M->E = M->dorefl ? emiss_eval(T, kLutReflFront, rough, dfresnel_table_index(M->sp.eta), cosNO)
+                     : 0.0f;
+}
+
+// SheenLobe constructor, Conty-Kulla mode (MTX/bsdf_sheen_impl.h:19-48)
+OSLMI_DEV void mx_sheen(MxLobe* M, V3 N, V3 albedo, float roughness, V3 wo)
+{
+    M->f      = frame_zx(visible_normal(wo, N, N), wo);
+    M->wol    = M->f.local(wo);
+    M->tint   = albedo;
+    M->dorefl = 1;
+    const float s_rough = regularize_roughness(CLAMP(roughness, 0.0f, 1.0f));
+    M->a                = CLAMP(smax(0.06f, s_rough), 0.06f, 1.0f);
+    const float cosNO   = CLAMP(vdot(M->f.Z, wo), 0.0f, 1.0f);
+    M->E                = 1 - smin(conty_kulla_albedo(M->a, cosNO) * pmax(M->tint), 1.0f);
+}
+
+// eval_impl at wi in the lobe frame (wo: the built one)
+OSLMI_DEV Sample mx_eval_local(const MxLobe& M, int kind, V3 wi)
+{
+    if (kind == kMxSheen) {
+        if (!(wi.z > 0 && M.wol.z >= 0))
+            return zero_sample();
+        return sheen_mf_eval(M.a, M.tint, M.wol, wi);
+    }
+    if (!M.dorefl)
+        return zero_sample();
+    Sample s = eval_turquin(M.sp, M.wol, wi);
+    s.weight = s.weight * (s.wi.z > 0 ? M.tint : V3(0.0f));
+    return s;
+}
+
+// sample_impl's direction in the lobe frame (false: no sample); the
+// sample's weight and pdf are mx_eval_local there
+OSLMI_DEV bool mx_sample_dir(const MxLobe& M, int kind, float rx, float ry, V3* wl)
+{
+    if (kind == kMxSheen) {
+        *wl = sample_uniform_hemisphere(rx, ry);
+        return true;
+    }
+    if (!M.dorefl || M.wol.z <= 0)
+        return false;
+    const V3 m = ggx_sample_bvndf<TrigFast>(M.sp.d, M.wol, rx, ry);
+    *wl        = reflect(M.wol, m);
+    return !(wl->z <= 0 || MAX_ABS_XYZ(*wl) < kEPSILON);
+}
+
+// albedo_impl (CompositeBSDF::prepare) and filter_o (layer opacity)
+OSLMI_HD V3 mx_albedo(const MxLobe& M) { return M.tint * (1 - M.E); }
+OSLMI_HD float mx_filter_o(const MxLobe& M) { return M.E; }
+
 }  // namespace mtx
 }  // namespace oslmi

@@ -195,12 +195,24 @@ OSLMI_DEV Dv triple(const Df& a, const Df& b, const Df& c) { return make_dv(a, b
 // group (the number of closure constructors it can reach, at most 8);
 // primitives make one-lobe lists, ADD concatenates, MUL scales.
 typedef LobeList<8> Closure;
+#ifdef OSLMI_MX_CLOSURES
+// a component (not the null closure); as the top of a layer, a lobe other
+// than a MaterialX one is opaque (evaluate_layer_opacity's default case)
+OSLMI_DEV void mark_component(LobeList<1>& c)
+{
+    c.nn = 1;
+    c.op = V3(1.0f);
+}
+#endif
 OSLMI_DEV LobeList<1> closure_diffuse(V3 N)
 {
     LobeList<1> c;
     Lobe l;
     l.id = kClosureDiffuse; l.weight = V3(1.0f); l.N = N; l.alpha = 0.0f; l.eta = 0.0f;
     c.add(l);
+#ifdef OSLMI_MX_CLOSURES
+    mark_component(c);
+#endif
     return c;
 }
 OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
@@ -209,6 +221,9 @@ OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
     Lobe l;
     l.id = kClosureMicrofacetGGX; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
     c.add(l);
+#ifdef OSLMI_MX_CLOSURES
+    mark_component(c);
+#endif
     return c;
 }
 OSLMI_DEV LobeList<1> closure_microfacet_beckmann(V3 N, float alpha, float eta)
@@ -217,30 +232,161 @@ OSLMI_DEV LobeList<1> closure_microfacet_beckmann(V3 N, float alpha, float eta)
     Lobe l;
     l.id = kClosureMicrofacetBeckmann; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
     c.add(l);
+#ifdef OSLMI_MX_CLOSURES
+    mark_component(c);
+#endif
     return c;
 }
-// color * closure (a MUL node): scales every lobe weight
+// color * closure (a MUL node): scales every lobe weight. A zero weight
+// gives the null closure (osl_mul_closure_color, opclosure.cpp:40-60); the
+// layer walks see the difference, lobe lists without MaterialX closures
+// do not (a zero-weight lobe adds nothing to an eval or a sample), so only
+// kernels with MaterialX closures pay for the test.
 template<int K> OSLMI_DEV LobeList<K> mul(const LobeList<K>& c, V3 w)
 {
     LobeList<K> r = c;
+#ifdef OSLMI_MX_CLOSURES
+    if (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f)
+        return LobeList<K>();
+#endif
 #pragma unroll
     for (int i = 0; i < K; ++i)
         if (i < r.n)
             r.l[i].weight = r.l[i].weight * w;
+#ifdef OSLMI_MX_CLOSURES
+    r.op = w * r.op;
+#endif
     return r;
 }
 template<int K> OSLMI_DEV LobeList<K> mul(V3 w, const LobeList<K>& c) { return mul(c, w); }
-// ADD node: lobes of a then b (depth-first order of process_bsdf_closure)
+// ADD node: lobes of a then b (depth-first order of process_bsdf_closure);
+// a null operand gives the other (osl_add_closure_closure)
 template<int KA, int KB>
 OSLMI_DEV LobeList<(KA + KB < 8 ? KA + KB : 8)> add(const LobeList<KA>& a, const LobeList<KB>& b)
 {
     LobeList<(KA + KB < 8 ? KA + KB : 8)> r(a);
+#ifdef OSLMI_MX_CLOSURES
+    // the walks visit A, then B (queued), unless they end inside A
+    if (!a.nn)
+        return LobeList<(KA + KB < 8 ? KA + KB : 8)>(b);
+    if (!b.nn)
+        return r;
+    if (!a.tr) {
+#pragma unroll
+        for (int i = 0; i < KB; ++i)
+            if (i < b.n)
+                r.add(b.l[i]);
+        r.tr = b.tr;
+    }
+    if (!a.opend) {
+        r.op    = b.op;
+        r.opend = b.opend;
+    }
+#else
 #pragma unroll
     for (int i = 0; i < KB; ++i)
         if (i < b.n)
             r.add(b.l[i]);
+#endif
     return r;
 }
+
+#ifdef OSLMI_MX_CLOSURES
+// MaterialX closures (stdosl.h dielectric_bsdf, generalized_schlick_bsdf,
+// sheen_bsdf), built at wo with the energy tables T. Their opacity as the
+// top of a layer is 1 - filter_o (evaluate_layer_opacity,
+// src/testrender/shading.cpp:1242-1268).
+OSLMI_DEV LobeList<1> closure_mx_lobe(int id, V3 N)
+{
+    LobeList<1> c;
+    Lobe l;
+    l.id     = id;
+    l.weight = V3(1.0f);
+    l.N      = N;
+    l.alpha  = 0.0f;
+    l.eta    = 0.0f;
+    c.add(l);
+    c.nn = 1;
+    return c;
+}
+OSLMI_DEV LobeList<1> closure_mx_dielectric(V3 N, V3 U, V3 refl_tint, float rx, float ry, float ior,
+                                            V3 wo, const float* T)
+{
+    LobeList<1> c = closure_mx_lobe(kClosureMxDielectric, N);
+    mtx::mx_micro(&c.l[0].mx, mtx::kMxDielectric, N, U, refl_tint, rx, ry, ior, V3(0.0f), V3(0.0f),
+                  0.0f, wo, T);
+    c.op = V3(1.0f) * (V3(1.0f) - V3(mtx::mx_filter_o(c.l[0].mx)));
+    return c;
+}
+OSLMI_DEV LobeList<1> closure_mx_schlick(V3 N, V3 U, V3 refl_tint, float rx, float ry, V3 F0, V3 F90,
+                                         float exponent, V3 wo, const float* T)
+{
+    LobeList<1> c = closure_mx_lobe(kClosureMxSchlick, N);
+    mtx::mx_micro(&c.l[0].mx, mtx::kMxSchlick, N, U, refl_tint, rx, ry, 0.0f, F0, F90, exponent, wo,
+                  T);
+    c.op = V3(1.0f) * (V3(1.0f) - V3(mtx::mx_filter_o(c.l[0].mx)));
+    return c;
+}
+OSLMI_DEV LobeList<1> closure_mx_sheen(V3 N, V3 albedo, float roughness, V3 wo)
+{
+    LobeList<1> c = closure_mx_lobe(kClosureMxSheen, N);
+    mtx::mx_sheen(&c.l[0].mx, N, albedo, roughness, wo);
+    c.op = V3(1.0f) * (V3(1.0f) - V3(mtx::mx_filter_o(c.l[0].mx)));
+    return c;
+}
+// layer(top, base): process_bsdf_closure's MX_LAYER_ID
+// (src/testrender/shading.cpp:1645-1662): the top's lobes as they are, the
+// base's scaled by 1 - clamp(opacity(top), 0, 1) and left out when that is
+// black. As the top of another layer, its opacity is the base's (walked
+// last), or the layer's own weight when the base is null.
+OSLMI_DEV float clamp01_nan0(float v)
+{
+    if (!(0.0f <= v))
+        v = 0.0f;
+    if (!(v <= 1.0f))
+        v = 1.0f;
+    return v;
+}
+template<int KT, int KB>
+OSLMI_DEV LobeList<(KT + KB < 8 ? KT + KB : 8)> layer(const LobeList<KT>& top, const LobeList<KB>& base)
+{
+    LobeList<(KT + KB < 8 ? KT + KB : 8)> r(top);
+    r.nn = 1;
+    // process_bsdf_closure: the base is queued (when its weight is not
+    // black) and visited after the top, unless the walk ended in the top
+    const V3 o  = top.nn ? top.op : V3(0.0f);
+    const V3 bw = V3(1.0f) - V3(clamp01_nan0(o.x), clamp01_nan0(o.y), clamp01_nan0(o.z));
+    r.tr        = 0;
+    if (!top.tr && !(bw.x == 0.0f && bw.y == 0.0f && bw.z == 0.0f)) {
+        if (!base.nn)
+            r.tr = 1;  // a queued null base ends the walk
+#pragma unroll
+        for (int i = 0; i < KB; ++i)
+            if (i < base.n) {
+                Lobe x   = base.l[i];
+                x.weight = bw * x.weight;
+                r.add(x);
+            }
+        if (base.nn)
+            r.tr = base.tr;
+    } else if (top.tr) {
+        r.tr = 1;
+    }
+    // evaluate_layer_opacity: the base is always queued; its opacity, or
+    // the layer's own weight when it is null (the walk ends there)
+    if (top.nn && top.opend) {
+        r.op    = top.op;
+        r.opend = 1;
+    } else if (base.nn) {
+        r.op    = base.op;
+        r.opend = base.opend;
+    } else {
+        r.op    = V3(1.0f);
+        r.opend = 1;
+    }
+    return r;
+}
+#endif
 
 }  // namespace rt
 }  // namespace oslmi

@@ -99,7 +99,7 @@ int build_code_object(const ShadingSys& ss, const std::string& src, const std::s
     std::string hdrs;
     for (const char* h : { "oslmi_rt.h", "oslmi_types.h", "oslmi_noise.h", "oslmi_gabor.h",
                            "oslmi_math.h", "oslmi_color.h", "oslmi_closure.h", "oslmi_groupargs.h",
-                           "oslmi_matrix.h" }) {
+                           "oslmi_matrix.h", "oslmi_mtx.h" }) {
         std::vector<char> c;
         read_file(ss.include_dir + "/" + h, c);
         hdrs.append(c.begin(), c.end());
@@ -176,6 +176,12 @@ int make_group_args(const ShadingSys& ss, const Group& g, const oslmi_batched_gl
         if (int rc = device_colorsys(&cs, allocate))
             return rc;
         A.colorsys = cs;
+    }
+    if (g.uses_mx) {
+        const float* lut = nullptr;
+        if (int rc = mtx::device_tables(&lut, allocate))
+            return rc;
+        A.bsdl = lut;
     }
     if (g.uses_xforms) {
         if (!allocate && !ss.xf.device)

@@ -426,6 +426,7 @@ __device__ __forceinline__ int route_class(const Domain& dom, unsigned long long
 // Per-wave queue of network queries in LDS: a ring of QCAP slots of 16
 // floats (14 raw inputs, the query index as bits, padding).
 constexpr int QCAP = 96, QSLOT = 16, WAVES_PER_BLOCK = 4;
+static_assert(QSLOT == 16, "slot reads index with k & (QSLOT - 1)");
 // Per-wave fallback list in LDS (flushed to the global list when full).
 constexpr int FBCAP = 64;
 
@@ -449,12 +450,17 @@ __device__ __forceinline__ void run_tiles(const float* __restrict__ ring, int he
     for (int u = 0; u < TILES; u++) {
         const int e     = u * 16 + r;
         const float* sl = ring + ((head + e) % QCAP) * QSLOT;
-        qi[u]           = e < count ? __float_as_int(sl[14]) : -1;
+        // the whole slot is read unconditionally (always inside the ring)
+        // and masked afterwards: conditional reads made the compiler wait
+        // for every outstanding global load, the next chunk's included
+        const int qv = __float_as_int(sl[14]);
+        qi[u]        = e < count ? qv : -1;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             // layer-1 operand: this lane group's 8 inputs (normalised)
-            const int k = 8 * g + j;
-            x[u][j] = (bf16)(k < NIN && e < count ? (sl[k < NIN ? k : 0] - noff[j]) * nscl[j] : 0.0f);
+            const int k   = 8 * g + j;
+            const float v = sl[k & (QSLOT - 1)];  // inside the slot; k >= NIN masked
+            x[u][j] = (bf16)(k < NIN && e < count ? (v - noff[j]) * nscl[j] : 0.0f);
         }
     }
     f32x4 a[TILES][4];

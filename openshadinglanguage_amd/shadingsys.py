@@ -254,6 +254,22 @@ def build_layered3(ss=None, outputs=()):
     return ss, g
 
 
+def build_mx_layered(ss=None, params=None):
+    """The closure workload's layered MaterialX material as a one-layer
+    group (shaders/mx_layered.osl): per-point parameters in the globals P
+    (coat ior, coat roughness, sheen roughness), dPdu (F0), dPdv (F90);
+    params: instance values of coat_tint, base_tint, sheen_albedo,
+    base_exponent."""
+    ss = ss or ShadingSystem()
+    ss.load_shader_file("mx_layered", os.path.join(SHADER_DIR, "mx_layered.oso"))
+    g = ss.ShaderGroupBegin("mx_layered")
+    for name, (typ, value) in (params or {}).items():
+        ss.Parameter(g, name, typ, value)
+    ss.Shader(g, "surface", "mx_layered", "surface")
+    ss.ShaderGroupEnd(g)
+    return ss, g
+
+
 def build_generated_group(ss, spec, name, outputs=("fout", "cout")):
     """A group from groupgen's (layers, connections, ...) description:
     layers [(shadername, layername, oso_text)], the last one the surface."""

@@ -111,6 +111,10 @@ float beckmann_D(V3 Hr, float xa, float ya)
 
 BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
 {
+    if (l.kind == kLobeMx) {
+        const mtx::Sample s = mtx::mx_eval(l.mx, wo, wi);
+        return { s.weight, s.pdf };
+    }
     if (l.kind == kLobeDiffuse) {
         const float pdf = std::max<float>(dot(l.N, wi), 0.0f) * float(M_1_PI);
         return { V3(1.0f), pdf };
@@ -206,6 +210,10 @@ V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
 BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
 {
     BsdfDirSample r = { V3(0.0f), V3(0.0f), 0.0f };
+    if (l.kind == kLobeMx) {
+        const mtx::Sample s = mtx::mx_sample(l.mx, wo, rx, ry);
+        return { s.wi, s.weight, s.pdf };
+    }
     if (l.kind == kLobeDiffuse) {
         to_unit_disk(rx, ry);
         const float cos_theta = sqrtf(std::max<float>(1 - rx * rx - ry * ry, 0.0f));
@@ -239,6 +247,8 @@ BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
 
 V3 lobe_albedo(const Lobe& l, V3 wo)
 {
+    if (l.kind == kLobeMx)
+        return mtx::mx_albedo(l.mx);
     if (l.kind == kLobeDiffuse)
         return V3(1.0f);
     float fr = fresnel_dielectric(dot(l.N, wo), l.eta);

@@ -1,11 +1,12 @@
 // CPU oracle: renderer-side closure evaluation (testrender).
 // TEST INFRASTRUCTURE ONLY (see oracle_core.h).
 #pragma once
+#include "mtx.h"
 #include "oracle_core.h"
 
 namespace oref {
 
-enum LobeKind { kLobeDiffuse = 0, kLobeGGXRefl = 1, kLobeBeckmannRefl = 2 };
+enum LobeKind { kLobeDiffuse = 0, kLobeGGXRefl = 1, kLobeBeckmannRefl = 2, kLobeMx = 3 };
 
 struct Lobe {
     int kind;
@@ -13,6 +14,7 @@ struct Lobe {
     V3 N;
     float alpha;  // microfacet xalpha = yalpha
     float eta;
+    mtx::MxLobe mx;  // kLobeMx: a MaterialX lobe built at the shading point's wo
 };
 
 struct BsdfSample {

@@ -11,6 +11,7 @@
 #include "../noise.cpp"
 #include "../gabor.cpp"
 #include "../color.cpp"
+#include "../mtx.cpp"
 #include "../closure.cpp"
 #include "../shade.cpp"
 

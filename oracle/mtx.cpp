@@ -351,27 +351,46 @@ float microlobe_roughness(float roughness_x, float roughness_y, float* aniso, bo
     return regularize_roughness(sqrtf(ax / (1 + *aniso)));
 }
 
-void dielectric_lobe(MicroLobe* L, const LayeredParams& p, V3 wo)
+// DielectricLobe (kind 0) / SchlickLobe (kind 1) constructors at wo
+// (MTX/bsdf_dielectric_impl.h:210-262, MTX/bsdf_schlick_impl.h:14-60):
+// frame (visible normal, tangent U), anisotropic GGX, Fresnel, energies
+void micro_build(MicroLobe* L, int kind, V3 N, V3 U, V3 refl_tint, V3 refr_tint, float roughness_x,
+                 float roughness_y, float ior, V3 F0, V3 F90, float exponent, V3 wo)
 {
-    L->frame     = frame_zx(visible_normal(wo, p.N, p.N), V3(0.0f));
-    L->refl_tint = p.coat_refl_tint;
-    L->refr_tint = p.coat_refr_tint;
-    L->dorefl    = MAX_RGB(p.coat_refl_tint) > 0;
-    L->dorefr    = MAX_RGB(p.coat_refr_tint) > 0;
+    L->frame     = frame_zx(visible_normal(wo, N, N), U);
+    L->refl_tint = refl_tint;
+    L->refr_tint = refr_tint;
+    L->dorefl    = MAX_RGB(refl_tint) > 0;
+    L->dorefr    = MAX_RGB(refr_tint) > 0;
     float aniso;
     bool flip;
-    const float roughness = microlobe_roughness(p.coat_roughness, p.coat_roughness, &aniso, &flip);
+    const float roughness = microlobe_roughness(roughness_x, roughness_y, &aniso, &flip);
     const float cosNO     = dot(L->frame.Z, wo);
-    const float IOR       = CLAMP(p.coat_ior, kIOR_MIN, kIOR_MAX);
     Fresnel f;
-    f.kind        = 0;
-    f.eta         = dfresnel_eta(IOR / 1.0f, false);
-    f.exponent    = 0;
-    f.tir_cos     = 0;
-    L->E_ms       = 0;
-    L->spec.d     = ggx_make(roughness, aniso, flip);
-    L->spec.f     = f;
-    L->spec.E_ms  = 0;
+    if (kind == 0) {
+        const float IOR = CLAMP(ior, kIOR_MIN, kIOR_MAX);
+        f.kind          = 0;
+        f.eta           = dfresnel_eta(IOR / 1.0f, false);
+        f.exponent      = 0;
+        f.tir_cos       = 0;
+    } else {
+        const float avg_F0         = CLAMP((F0.x + F0.y + F0.z) * (1.0f / 3), 0.0f, 0.99f);
+        const float sqrt_F0        = sqrtf(avg_F0);
+        const float refraction_ior = (1 + sqrt_F0) / (1 - sqrt_F0);
+        f.kind     = 1;
+        f.eta      = dfresnel_eta(refraction_ior, false);
+        f.F0       = V3(smax(0.0f, smin(1.0f, F0.x)), smax(0.0f, smin(1.0f, F0.y)),
+                        smax(0.0f, smin(1.0f, F0.z)));
+        f.F90      = V3(smax(0.0f, smin(1.0f, F90.x)), smax(0.0f, smin(1.0f, F90.y)),
+                        smax(0.0f, smin(1.0f, F90.z)));
+        f.exponent = exponent;
+        const float zero = 0.0f;
+        f.tir_cos  = f.eta >= 1 ? zero : sqrtf(1 - SQR(f.eta));
+    }
+    L->E_ms      = 0;
+    L->spec.d    = ggx_make(roughness, aniso, flip);
+    L->spec.f    = f;
+    L->spec.E_ms = 0;
     if (!L->dorefr)
         L->spec.E_ms = emiss_eval_t(luts(), kLutMiniGGX, roughness, 0.0f, cosNO);
     if (L->dorefl && !L->dorefr)
@@ -379,39 +398,16 @@ void dielectric_lobe(MicroLobe* L, const LayeredParams& p, V3 wo)
     L->roughness = roughness;
 }
 
+void dielectric_lobe(MicroLobe* L, const LayeredParams& p, V3 wo)
+{
+    micro_build(L, 0, p.N, V3(0.0f), p.coat_refl_tint, p.coat_refr_tint, p.coat_roughness,
+                p.coat_roughness, p.coat_ior, V3(0.0f), V3(0.0f), 0.0f, wo);
+}
+
 void schlick_lobe(MicroLobe* L, const LayeredParams& p, V3 wo)
 {
-    L->frame     = frame_zx(visible_normal(wo, p.N, p.N), V3(0.0f));
-    L->refl_tint = p.base_refl_tint;
-    L->refr_tint = p.base_refr_tint;
-    L->dorefl    = MAX_RGB(p.base_refl_tint) > 0;
-    L->dorefr    = MAX_RGB(p.base_refr_tint) > 0;
-    float aniso;
-    bool flip;
-    const float roughness = microlobe_roughness(p.base_roughness, p.base_roughness, &aniso, &flip);
-    const float cosNO     = dot(L->frame.Z, wo);
-    const float avg_F0 = CLAMP((p.base_f0.x + p.base_f0.y + p.base_f0.z) * (1.0f / 3), 0.0f, 0.99f);
-    const float sqrt_F0        = sqrtf(avg_F0);
-    const float refraction_ior = (1 + sqrt_F0) / (1 - sqrt_F0);
-    Fresnel f;
-    f.kind     = 1;
-    f.eta      = dfresnel_eta(refraction_ior, false);
-    f.F0       = V3(smax(0.0f, smin(1.0f, p.base_f0.x)), smax(0.0f, smin(1.0f, p.base_f0.y)),
-                    smax(0.0f, smin(1.0f, p.base_f0.z)));
-    f.F90      = V3(smax(0.0f, smin(1.0f, p.base_f90.x)), smax(0.0f, smin(1.0f, p.base_f90.y)),
-                    smax(0.0f, smin(1.0f, p.base_f90.z)));
-    f.exponent = p.base_exponent;
-    const float zero = 0.0f;
-    f.tir_cos  = f.eta >= 1 ? zero : sqrtf(1 - SQR(f.eta));
-    L->E_ms    = 0;
-    L->spec.d  = ggx_make(roughness, aniso, flip);
-    L->spec.f  = f;
-    L->spec.E_ms = 0;
-    if (!L->dorefr)
-        L->spec.E_ms = emiss_eval_t(luts(), kLutMiniGGX, roughness, 0.0f, cosNO);
-    if (L->dorefl && !L->dorefr)
-        L->E_ms = emiss_eval_t(luts(), kLutReflFront, roughness, dfresnel_table_index(f.eta), cosNO);
-    L->roughness = roughness;
+    micro_build(L, 1, p.N, V3(0.0f), p.base_refl_tint, p.base_refr_tint, p.base_roughness,
+                p.base_roughness, 0.0f, p.base_f0, p.base_f90, p.base_exponent, wo);
 }
 
 Sample microlobe_eval_local(const MicroLobe& L, V3 wo, V3 wi)
@@ -454,11 +450,12 @@ Sample sheen_mf_eval(float a, V3 wo, V3 wi)
     return Sample { wi, V3(D * G2 * 0.5f * kPI / cosNO), 0.5f * kONEOVERPI, 0.0f };
 }
 
-void sheen_lobe(SheenLobeS* L, const LayeredParams& p, V3 wo)
+// SheenLobe constructor, Conty-Kulla mode (MTX/bsdf_sheen_impl.h:19-48)
+void sheen_build(SheenLobeS* L, V3 N, V3 albedo, float roughness, V3 wo)
 {
-    L->frame      = frame_zx(visible_normal(wo, p.N, p.N), wo);
-    L->roughness  = regularize_roughness(CLAMP(p.sheen_roughness, 0.0f, 1.0f));
-    L->tint       = p.sheen_albedo;
+    L->frame      = frame_zx(visible_normal(wo, N, N), wo);
+    L->roughness  = regularize_roughness(CLAMP(roughness, 0.0f, 1.0f));
+    L->tint       = albedo;
     L->Emiss      = 1.0f;
     L->backfacing = false;
     const float sheen_alpha = smax(0.06f, L->roughness);
@@ -468,6 +465,10 @@ void sheen_lobe(SheenLobeS* L, const LayeredParams& p, V3 wo)
         L->Emiss = 1;
     else
         L->Emiss = 1 - smin(conty_kulla_albedo(L->a, cosNO) * pmax(L->tint), 1.0f);
+}
+void sheen_lobe(SheenLobeS* L, const LayeredParams& p, V3 wo)
+{
+    sheen_build(L, p.N, p.sheen_albedo, p.sheen_roughness, wo);
 }
 Sample sheen_eval_local(const SheenLobeS& L, V3 wo, V3 wi)
 {
@@ -680,6 +681,59 @@ struct LayeredBsdf {
     SheenLobeS sheen;
     MicroLobe coat, base;
 };
+
+// ---- one MaterialX lobe of a closure list ---------------------------------
+namespace {
+struct MxL {
+    int kind;  // 0 dielectric, 1 generalized Schlick, 2 sheen
+    MicroLobe m;
+    SheenLobeS s;
+};
+static_assert(sizeof(MxL) <= sizeof(MxLobe::state), "MxLobe storage");
+const MxL* mxl(const MxLobe& L) { return reinterpret_cast<const MxL*>(L.state); }
+}  // namespace
+
+void mx_build(MxLobe* L, const MxParams& p, V3 wo)
+{
+    MxL* x = reinterpret_cast<MxL*>(L->state);
+    *x     = MxL();
+    x->kind = p.kind;
+    if (p.kind == 2)
+        sheen_build(&x->s, p.N, p.tint, p.rx, wo);
+    else
+        micro_build(&x->m, p.kind, p.N, p.U, p.tint, V3(0.0f), p.rx, p.ry, p.ior, p.F0, p.F90,
+                    p.exponent, wo);
+}
+// BSDL_WRAP::eval / sample (src/testrender/shading.cpp:88-101)
+Sample mx_eval(const MxLobe& L, V3 wo, V3 wi)
+{
+    const MxL* x = mxl(L);
+    Sample s;
+    if (x->kind == 2)
+        s = sheen_eval_local(x->s, x->s.frame.local(wo), x->s.frame.local(wi));
+    else
+        s = microlobe_eval_local(x->m, x->m.frame.local(wo), x->m.frame.local(wi));
+    return Sample { wi, s.weight, s.pdf, s.roughness };
+}
+Sample mx_sample(const MxLobe& L, V3 wo, float rx, float ry)
+{
+    const MxL* x   = mxl(L);
+    const Frame& f = x->kind == 2 ? x->s.frame : x->m.frame;
+    Sample s       = x->kind == 2 ? sheen_sample_local(x->s, f.local(wo), rx, ry)
+                                  : microlobe_sample_local(x->m, f.local(wo), rx, ry);
+    return Sample { f.world(s.wi), s.weight, s.pdf, s.roughness };
+}
+// albedo_impl, filter_o (MTX/bsdf_*_decl.h)
+V3 mx_albedo(const MxLobe& L)
+{
+    const MxL* x = mxl(L);
+    return x->kind == 2 ? x->s.tint * (1 - x->s.Emiss) : x->m.refl_tint * (1 - x->m.E_ms);
+}
+float mx_filter_o(const MxLobe& L)
+{
+    const MxL* x = mxl(L);
+    return x->kind == 2 ? x->s.Emiss : x->m.E_ms;
+}
 
 LayeredBsdf* layered_new() { return new LayeredBsdf(); }
 void layered_free(LayeredBsdf* b) { delete b; }

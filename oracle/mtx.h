@@ -34,6 +34,27 @@ struct Sample {
     float roughness;
 };
 
+// One MaterialX closure lobe (testrender's BSDL_WRAP around libbsdl's
+// DielectricLobe / SchlickLobe / SheenLobe, front facing, path roughness
+// 0, reflection only), built at wo: parameters, then opaque state.
+struct MxParams {
+    int kind;  // 0 dielectric_bsdf, 1 generalized_schlick_bsdf, 2 sheen_bsdf
+    V3 N, U;
+    V3 tint;   // reflection tint; sheen albedo
+    float rx, ry;  // roughness_x, roughness_y; sheen: rx = roughness
+    float ior;     // dielectric
+    V3 F0, F90;    // Schlick
+    float exponent;
+};
+struct MxLobe {
+    alignas(16) unsigned char state[384];
+};
+void mx_build(MxLobe* L, const MxParams& p, V3 wo);
+Sample mx_eval(const MxLobe& L, V3 wo, V3 wi);              // world directions
+Sample mx_sample(const MxLobe& L, V3 wo, float rx, float ry);  // world wi
+V3 mx_albedo(const MxLobe& L);
+float mx_filter_o(const MxLobe& L);
+
 // Per-point parameters of the layered material
 // layer(sheen_bsdf, layer(dielectric_bsdf, generalized_schlick_bsdf)).
 struct LayeredParams {

@@ -227,7 +227,23 @@ bool parse(const std::string& text, Master& m, std::string& err)
 struct Val {
     int i      = 0;
     float f[16] = { 0 };  // float: f[0]; triple: f[0..2]; matrix: f[0..15]
-    Composite cl;         // closure: lobe list (weights, not yet prepared)
+    int cl = -1;          // closure: a node of the point's closure tree (-1: null)
+};
+
+// Closure values are trees like the reference's ClosureColor (ADD, MUL
+// and component nodes; src/include/OSL/oslclosure.h:41-145), built by the
+// closure ops with osl_add_closure_closure / osl_mul_closure_color's
+// null rules (opclosure.cpp:19-80) and walked after the shader by
+// testrender's process_bsdf_closure.
+enum CType { kCAdd, kCMul, kCComp };
+enum CComp { kCompClassic, kCompMx, kCompLayer };
+struct CNode {
+    int type = kCComp;
+    int a = -1, b = -1;  // ADD operands; MUL operand (a); layer top (a), base (b)
+    V3 w   = V3(1.0f);   // MUL weight
+    int comp = kCompClassic;
+    Lobe lobe {};        // classic lobe parameters
+    mtx::MxParams mx {};  // MaterialX lobe parameters
 };
 
 struct Layer {
@@ -263,9 +279,10 @@ struct Group {
 struct Exec {
     const Group& G;
     std::vector<std::vector<Val>> vals;  // [layer][sym]
-    V3 P, N;
+    V3 P, N, dPdu, dPdv;
     bool has_ci = false;
-    Composite ci;
+    int ci = -1;
+    std::vector<CNode> cn;  // the point's closure nodes
 
     explicit Exec(const Group& g) : G(g) {}
 
@@ -356,6 +373,10 @@ struct Exec {
                     setT(L, (int)s, P);
                 else if (y.name == "N" || y.name == "Ng")
                     setT(L, (int)s, N);
+                else if (y.name == "dPdu")
+                    setT(L, (int)s, dPdu);
+                else if (y.name == "dPdv")
+                    setT(L, (int)s, dPdv);
                 continue;
             }
             if (y.base == kInt && !y.fdef.empty())
@@ -472,19 +493,37 @@ struct Exec {
         if (o == "add" || o == "sub" || o == "mul" || o == "div") {
             if (rb == kClosure) {
                 if (o == "add") {
-                    Composite c = vals[L][a(1)].cl;
-                    const Composite& b = vals[L][a(2)].cl;
-                    for (int k = 0; k < b.n; ++k)
-                        c.add(b.lobes[k]);
-                    vals[L][r].cl = c;
+                    // osl_add_closure_closure: a null operand gives the other
+                    const int x = vals[L][a(1)].cl, y = vals[L][a(2)].cl;
+                    if (x < 0 || y < 0) {
+                        vals[L][r].cl = x < 0 ? y : x;
+                    } else {
+                        CNode nd;
+                        nd.type = kCAdd;
+                        nd.a    = x;
+                        nd.b    = y;
+                        cn.push_back(nd);
+                        vals[L][r].cl = (int)cn.size() - 1;
+                    }
                 } else {
+                    // osl_mul_closure_color / _float: null for a null closure
+                    // or a zero weight, the closure itself for weight one
                     const int ci_ = B(L, a(1)) == kClosure ? a(1) : a(2);
                     const int wi  = B(L, a(1)) == kClosure ? a(2) : a(1);
-                    Composite c   = vals[L][ci_].cl;
+                    const int x   = vals[L][ci_].cl;
                     const V3 w    = T(L, wi);
-                    for (int k = 0; k < c.n; ++k)
-                        c.lobes[k].weight = c.lobes[k].weight * w;
-                    vals[L][r].cl = c;
+                    if (x < 0 || (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f)) {
+                        vals[L][r].cl = -1;
+                    } else if (w.x == 1.0f && w.y == 1.0f && w.z == 1.0f) {
+                        vals[L][r].cl = x;
+                    } else {
+                        CNode nd;
+                        nd.type = kCMul;
+                        nd.a    = x;
+                        nd.w    = w;
+                        cn.push_back(nd);
+                        vals[L][r].cl = (int)cn.size() - 1;
+                    }
                 }
                 return;
             }
@@ -732,22 +771,56 @@ struct Exec {
         }
         if (o == "closure") {
             const std::string cname = Sv(L, a(1));
-            Composite c;
-            Lobe l;
-            l.weight = V3(1.0f);
-            if (cname == "diffuse") {
-                l.kind  = kLobeDiffuse;
-                l.N     = T(L, a(2));
-                l.alpha = 0.0f;
-                l.eta   = 0.0f;
+            CNode nd;
+            nd.type = kCComp;
+            if (cname == "layer") {
+                // MX_LAYER_ID with its two closure parameters
+                nd.comp = kCompLayer;
+                nd.a    = vals[L][a(2)].cl;
+                nd.b    = vals[L][a(3)].cl;
+            } else if (cname == "dielectric_bsdf" || cname == "generalized_schlick_bsdf"
+                       || cname == "sheen_bsdf") {
+                // MaterialX closures, positional parameters as registered
+                // (MTX/bsdf_*_decl.h entry()); keyword ones change nothing
+                // the interpreter supports
+                nd.comp  = kCompMx;
+                mtx::MxParams& m = nd.mx;
+                m.N              = T(L, a(2));
+                if (cname == "sheen_bsdf") {
+                    m.kind = 2;
+                    m.tint = T(L, a(3));
+                    m.rx = m.ry = F(L, a(4));
+                } else {
+                    m.kind = cname == "dielectric_bsdf" ? 0 : 1;
+                    m.U    = T(L, a(3));
+                    m.tint = T(L, a(4));
+                    m.rx   = F(L, a(6));
+                    m.ry   = F(L, a(7));
+                    if (m.kind == 0) {
+                        m.ior = F(L, a(8));
+                    } else {
+                        m.F0       = T(L, a(8));
+                        m.F90      = T(L, a(9));
+                        m.exponent = F(L, a(10));
+                    }
+                }
             } else {
-                l.kind  = Sv(L, a(2)) == "beckmann" ? kLobeBeckmannRefl : kLobeGGXRefl;
-                l.N     = T(L, a(3));
-                l.alpha = F(L, a(5));
-                l.eta   = F(L, a(7));
+                Lobe& l  = nd.lobe;
+                l.weight = V3(1.0f);
+                if (cname == "diffuse") {
+                    l.kind  = kLobeDiffuse;
+                    l.N     = T(L, a(2));
+                    l.alpha = 0.0f;
+                    l.eta   = 0.0f;
+                } else {
+                    l.kind  = Sv(L, a(2)) == "beckmann" ? kLobeBeckmannRefl : kLobeGGXRefl;
+                    l.N     = T(L, a(3));
+                    l.alpha = F(L, a(5));
+                    l.eta   = F(L, a(7));
+                }
             }
-            c.add(l);
-            vals[L][r].cl = c;
+            cn.push_back(nd);
+            vals[L][r].cl = (int)cn.size() - 1;
             return;
         }
         fprintf(stderr, "osoint: unsupported op %s\n", o.c_str());
@@ -824,11 +897,121 @@ struct Exec {
         return kNext;
     }
 
+    // OIIO clamp on each channel
+    static float oclamp01(float a)
+    {
+        float v = a;
+        if (!(0.0f <= v))
+            v = 0.0f;
+        if (!(v <= 1.0f))
+            v = 1.0f;
+        return v;
+    }
+
+    // evaluate_layer_opacity (src/testrender/shading.cpp:1197-1272): the
+    // walk's weight when it stops (its stack pops a null pointer or runs
+    // empty); MaterialX lobes multiply 1 - filter_o at wo, other
+    // components count as opaque
+    V3 layer_opacity(int closure, V3 wo) const
+    {
+        if (closure < 0)
+            return V3(0.0f);
+        int stack[16];
+        V3 wstack[16];
+        int sp    = 0;
+        V3 weight = V3(1.0f);
+        while (closure >= 0) {
+            const CNode& nd = cn[closure];
+            if (nd.type == kCMul) {
+                weight  = weight * nd.w;
+                closure = nd.a;
+            } else if (nd.type == kCAdd) {
+                stack[sp]    = nd.b;
+                wstack[sp++] = weight;
+                closure      = nd.a;
+            } else {
+                const V3 w(1.0f);
+                if (nd.comp == kCompLayer) {
+                    closure      = nd.a;
+                    stack[sp]    = nd.b;
+                    wstack[sp++] = weight * w;
+                } else if (nd.comp == kCompMx) {
+                    mtx::MxLobe lobe;
+                    mtx::mx_build(&lobe, nd.mx, wo);
+                    weight  = weight * (w * (V3(1.0f) - V3(mtx::mx_filter_o(lobe))));
+                    closure = -1;
+                } else {
+                    closure = -1;
+                }
+            }
+            if (closure < 0 && sp > 0) {
+                closure = stack[--sp];
+                weight  = wstack[sp];
+            }
+        }
+        return weight;
+    }
+
+    // process_bsdf_closure (src/testrender/shading.cpp:1448-1667) over Ci at
+    // wo, path weight 1: the flattened lobe list, not yet prepared
+    Composite process(V3 wo) const
+    {
+        Composite c;
+        if (!has_ci)
+            return c;
+        int closure = ci;
+        int stack[16];
+        V3 wstack[16];
+        int sp    = 0;
+        V3 weight = V3(1.0f);
+        while (closure >= 0) {
+            const CNode& nd = cn[closure];
+            if (nd.type == kCMul) {
+                weight  = weight * nd.w;
+                closure = nd.a;
+            } else if (nd.type == kCAdd) {
+                stack[sp]    = nd.b;
+                wstack[sp++] = weight;
+                closure      = nd.a;
+            } else {
+                const V3 cw = weight * V3(1.0f);
+                closure     = -1;
+                if (nd.comp == kCompLayer) {
+                    const V3 op     = layer_opacity(nd.a, wo);
+                    const V3 base_w = weight * (V3(1.0f) - V3(oclamp01(op.x), oclamp01(op.y), oclamp01(op.z)));
+                    closure         = nd.a;
+                    weight          = cw;
+                    if (!(base_w.x == 0 && base_w.y == 0 && base_w.z == 0)) {
+                        stack[sp]    = nd.b;
+                        wstack[sp++] = base_w;
+                    }
+                } else if (nd.comp == kCompMx) {
+                    Lobe l {};
+                    l.kind   = kLobeMx;
+                    l.weight = cw;
+                    l.N      = nd.mx.N;
+                    mtx::mx_build(&l.mx, nd.mx, wo);
+                    c.add(l);
+                } else {
+                    Lobe l   = nd.lobe;
+                    l.weight = cw;
+                    c.add(l);
+                }
+            }
+            if (closure < 0 && sp > 0) {
+                closure = stack[--sp];
+                weight  = wstack[sp];
+            }
+        }
+        return c;
+    }
+
     void run()
     {
         vals.assign(G.layers.size(), {});
+        cn.clear();
         has_ci = false;
-        ci     = Composite();
+        ci     = -1;
         // Ci is one global of the shading point, shared by the layers: each
         // layer starts from the value the layers before it left
         auto ci_sym = [&](int L) {
@@ -939,9 +1122,9 @@ void oref_group_outputs(void* h, const char* fname, const char* cname)
 }
 
 namespace {
-void run_group(Group* g, const float* P, const float* N, const float* wo, const float* wi,
-               const float* rnd, int64_t n, float* out4, float* eval4, float* samp7,
-               int nthreads)
+void run_group(Group* g, const float* P, const float* N, const float* dPdu, const float* dPdv,
+               const float* wo, const float* wi, const float* rnd, int64_t n, float* out4,
+               float* eval4, float* samp7, int nthreads)
 {
     const int last = (int)g->layers.size() - 1;
     const int sf   = g->layers[last].m->find(g->fout);
@@ -949,8 +1132,10 @@ void run_group(Group* g, const float* P, const float* N, const float* wo, const 
     auto work = [&](int64_t b, int64_t e) {
         Exec ex(*g);
         for (int64_t i = b; i < e; ++i) {
-            ex.P = V3(P[i], P[n + i], P[2 * n + i]);
-            ex.N = V3(N[i], N[n + i], N[2 * n + i]);
+            ex.P    = V3(P[i], P[n + i], P[2 * n + i]);
+            ex.N    = V3(N[i], N[n + i], N[2 * n + i]);
+            ex.dPdu = dPdu ? V3(dPdu[i], dPdu[n + i], dPdu[2 * n + i]) : V3(0.0f);
+            ex.dPdv = dPdv ? V3(dPdv[i], dPdv[n + i], dPdv[2 * n + i]) : V3(0.0f);
             ex.run();
             out4[i]         = sf >= 0 ? ex.F(last, sf) : float(0.0f);
             const V3 c      = sc >= 0 ? ex.T(last, sc) : V3(0.0f);
@@ -958,8 +1143,8 @@ void run_group(Group* g, const float* P, const float* N, const float* wo, const 
             out4[2 * n + i] = c.y;
             out4[3 * n + i] = c.z;
             if (eval4 || samp7) {
-                Composite ci = ex.has_ci ? ex.ci : Composite();
                 const V3 o(wo[i], wo[n + i], wo[2 * n + i]);
+                Composite ci = ex.process(o);
                 ci.prepare(o);
                 if (eval4) {
                     const V3 w(wi[i], wi[n + i], wi[2 * n + i]);
@@ -1005,7 +1190,16 @@ void run_group(Group* g, const float* P, const float* N, const float* wo, const 
 void oref_group_run(void* h, const float* P, const float* N, const float* wo, const float* wi,
                     int64_t n, float* out4, float* eval4, int nthreads)
 {
-    run_group((Group*)h, P, N, wo, wi, nullptr, n, out4, eval4, nullptr, nthreads);
+    run_group((Group*)h, P, N, nullptr, nullptr, wo, wi, nullptr, n, out4, eval4, nullptr, nthreads);
+}
+
+// Both, with the globals dPdu and dPdv (3 planes each, or null: zero) and
+// either of eval4 / samp7 null to skip it.
+void oref_group_run_ex(void* h, const float* P, const float* N, const float* dPdu,
+                       const float* dPdv, const float* wo, const float* wi, const float* rnd,
+                       int64_t n, float* out4, float* eval4, float* samp7, int nthreads)
+{
+    run_group((Group*)h, P, N, dPdu, dPdv, wo, wi, rnd, n, out4, eval4, samp7, nthreads);
 }
 
 // The same, with Ci sampled (CompositeBSDF::sample) at wo with the random
@@ -1013,7 +1207,7 @@ void oref_group_run(void* h, const float* P, const float* N, const float* wo, co
 void oref_group_run_sample(void* h, const float* P, const float* N, const float* wo,
                            const float* rnd, int64_t n, float* out4, float* samp7, int nthreads)
 {
-    run_group((Group*)h, P, N, wo, nullptr, rnd, n, out4, nullptr, samp7, nthreads);
+    run_group((Group*)h, P, N, nullptr, nullptr, wo, nullptr, rnd, n, out4, nullptr, samp7, nthreads);
 }
 
 }  // extern "C"

@@ -452,6 +452,25 @@ def composite_sample(lobes, wo, rnd):
     return out
 
 
+def group_run_ex(group, P, N, wo, dPdu=None, dPdv=None, wi=None, rnd=None, nthreads=1):
+    """The interpreter group's outputs, with the globals dPdu / dPdv, and
+    Ci evaluated at (wo, wi) and / or sampled at (wo, rnd): (out4, eval4 or
+    None, samp7 or None)."""
+    L = group.L
+    L.oref_group_run_ex.argtypes = [ctypes.c_void_p] * 8 + [ctypes.c_int64] + [ctypes.c_void_p] * 3 + [
+        ctypes.c_int]
+    f = lambda x: None if x is None else np.ascontiguousarray(x, np.float32)  # noqa: E731
+    P, N, wo, dPdu, dPdv, wi, rnd = map(f, (P, N, wo, dPdu, dPdv, wi, rnd))
+    n = P.shape[1]
+    out = np.zeros((4, n), np.float32)
+    ev = np.zeros((4, n), np.float32) if wi is not None else None
+    samp = np.zeros((7, n), np.float32) if rnd is not None else None
+    q = lambda x: None if x is None else _p(x)  # noqa: E731
+    L.oref_group_run_ex(group.h, _p(P), _p(N), q(dPdu), q(dPdv), _p(wo), q(wi), q(rnd), n, _p(out),
+                        q(ev), q(samp), nthreads)
+    return out, ev, samp
+
+
 def group_run_sample(group, P, N, wo, rnd, nthreads=1):
     """The interpreter group's outputs and Ci sampled at (wo, rnd)."""
     L = group.L

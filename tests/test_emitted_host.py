@@ -1,0 +1,50 @@
+"""CPU check of the emitter's closure lowering: each MaterialX closure
+program's emitted kernel, compiled for the host (tests/hostkernel/
+harness.cpp defines the device qualifiers away and runs the grid-stride
+loop on one thread), evaluates its closure exactly as the oracle's
+interpreter does (closure trees flattened by testrender's walk). The
+kernel code and the interpreter differ in structure (lobe lists built
+eagerly at each closure op against a tree walked afterwards), so a slip
+in either (a null rule, an opacity, a walk that should stop) shows here
+without a GPU. The GPU runs the same sources (test_gpu_mx_closure.py)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import _mxprogs as X
+import _oracle as O
+from openshadinglanguage_amd import shadingsys as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "openshadinglanguage_amd", "csrc")
+LUTS = os.path.join(HERE, "..", "oracle", "build", "bsdl_luts.bin")
+
+
+@pytest.mark.parametrize("k", range(len(X.programs())), ids=[p[0] for p in X.programs()])
+def test_host_built_kernel_matches_interpreter(tmp_path, k):
+    name, o = X.programs()[k]
+    ss = S.ShadingSystem()
+    g = S.build_generated_group(ss, ([(name, "surface", o.text())], []), name, outputs=())
+    src = tmp_path / (name + ".hip")
+    src.write_text(ss.emit(g))
+    exe = tmp_path / "harness"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-I", CSRC,
+                           "-I", str(tmp_path), '-DKERNEL="%s.hip"' % name, "-o", str(exe),
+                           os.path.join(HERE, "hostkernel", "harness.cpp")])
+    inp = X.inputs(1500, 60 + k)
+    n = inp["P"].shape[1]
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as f:
+        f.write(np.int64(n).tobytes())
+        for key in ("P", "N", "dPdu", "dPdv", "wo", "wi"):
+            f.write(np.ascontiguousarray(inp[key], np.float32).tobytes())
+    fout = tmp_path / "out.bin"
+    O.mtx_tables()  # the oracle's table file exists once the oracle has loaded it
+    subprocess.check_call([str(exe), str(fin), str(fout), LUTS])
+    ev = np.fromfile(fout, np.float32).reshape(4, n)
+    gr = O.Group([(name, "surface", o.text())], [])
+    _, rev, _ = O.group_run_ex(gr, inp["P"], inp["N"], inp["wo"], dPdu=inp["dPdu"],
+                               dPdv=inp["dPdv"], wi=inp["wi"])
+    assert np.array_equal(ev, rev), (name, np.mean(ev == rev))

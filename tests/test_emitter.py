@@ -204,3 +204,55 @@ def test_closure_capacity_follows_dataflow():
     src = ss.emit(S.build_generated_group(ss, (layers, conns), "up"))
     assert "using Closure = LobeList<2>;" in src
     assert src.count("ci_ = ") == 2  # both layers hand Ci on
+
+
+def _mx_shader(name, kind, extra=()):
+    """A surface shader with one MaterialX closure; extra: keyword (name,
+    Oso-constant args) pairs appended to the closure call."""
+    import _mxprogs as X
+    w = X._W(name)
+    x, y, z = w.f
+    if kind == "dielectric":
+        args = ["N", w.zero_v(), w.cconst(1, 1, 1), w.zero_c(), x, x, w.fconst(1.5),
+                w.o.const("string", "ggx")]
+    elif kind == "dielectric_refr":
+        args = ["N", w.zero_v(), w.cconst(1, 1, 1), w.cconst(0.5, 0.5, 0.5), x, x, w.fconst(1.5),
+                w.o.const("string", "ggx")]
+        kind = "dielectric"
+    else:
+        args = ["N", w.cconst(1, 1, 1), z]
+    cname = "dielectric_bsdf" if kind == "dielectric" else "sheen_bsdf"
+    for key, val in extra:
+        args += [w.o.const("string", key), val(w)]
+    return w.done(w.closure(cname, *args))
+
+
+def test_materialx_closures_emit_and_reject_what_they_do_not_model():
+    """dielectric_bsdf / sheen_bsdf / layer compile to libbsdl lobes (the
+    kernel defines OSLMI_MX_CLOSURES); a transmission tint, an absorption
+    or a sheen mode other than a constant zero, and unknown keywords, are
+    rejected at emit time; labels and thin-film keywords pass."""
+    import _mxprogs as X
+    ss = S.ShadingSystem()
+    for name, o in X.programs():
+        g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), name, outputs=())
+        src = ss.emit(g)
+        assert "#define OSLMI_MX_CLOSURES 1" in src and "mxwo_" in src, name
+    ok = [_mx_shader("ok_d", "dielectric", (("label", lambda w: w.o.const("string", "coat")),
+                                            ("thinfilm_thickness", lambda w: w.fconst(0.0)))),
+          _mx_shader("ok_s", "sheen", (("mode", lambda w: w.o.const("int", 0)),))]
+    for o in ok:
+        g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), o.name, outputs=())
+        ss.emit(g)
+    bad = [(_mx_shader("b_refr", "dielectric_refr"), "transmission"),
+           (_mx_shader("b_abs", "dielectric", (("absorption", lambda w: w.cconst(0.2, 0, 0)),)),
+            "absorption"),
+           (_mx_shader("b_mode", "sheen", (("mode", lambda w: w.o.const("int", 1)),)), "mode"),
+           (_mx_shader("b_kw", "sheen", (("shininess", lambda w: w.fconst(1.0)),)), "shininess")]
+    for o, word in bad:
+        g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), o.name, outputs=())
+        with pytest.raises(OSLMIError, match=word):
+            ss.emit(g)
+    # classic groups are compiled without the MaterialX lobe state
+    ss2, g2 = S.build_layered3()
+    assert "OSLMI_MX_CLOSURES" not in ss2.emit(g2)
