@@ -81,7 +81,9 @@ void oslmi_shadingsys_destroy(oslmi_shadingsys* ss);
 /* ShadingSystem::attribute (oslexec.h:177-324): "colorspace", "jit_cache",
  * "include_dir", "commonspace" (the synonym of "common", default "world"),
  * "closure_sample" (0/1: surface kernels compiled afterwards also carry
- * CompositeBSDF::sample, for oslmi_execute_batch_sample), and
+ * CompositeBSDF::sample, for oslmi_execute_batch_sample),
+ * "closure_export" (0/1: they also export Ci's lobes, for
+ * oslmi_execute_batch_closures), and
  * "gpu_waves_per_eu" (0..8: the occupancy, in waves per SIMD, the emitted
  * kernels are register-allocated for; default 6, 0 leaves it to the
  * compiler). */
@@ -172,6 +174,31 @@ int oslmi_execute_batch_sample(oslmi_shadingsys* ss, oslmi_group* g,
  * uniform pointers and object2common / shader2common / Ci are opaque
  * renderer data here and are not read. */
 #if defined(__cplusplus)
+/* The same execution with the surface closure Ci handed out as testrender
+ * gets it from OSL: the list of components process_bsdf_closure passes to
+ * CompositeBSDF::add_bsdf (src/testrender/shading.cpp:1448-1667), each as
+ * (closure id, weight rgb, parameters) with testrender's ids and parameter
+ * order (register_closures, shading.cpp:185-296; ClosureIDs, shading.h:
+ * 25-61; oslexec.h:1041 register_closure). Layers are resolved into the
+ * weights, so MX_LAYER_ID does not appear. d_closures: 1 + max_lobes *
+ * OSLMI_CLOSURE_RECORD planes of n floats: plane 0 the count (at most
+ * max_lobes, 1..8); lobe k's record at planes 1 + k * OSLMI_CLOSURE_RECORD:
+ * id, weight r, g, b, then OSLMI_CLOSURE_PARAMS parameter floats
+ * (DIFFUSE_ID 3: N; MICROFACET_ID 8: distribution (0 ggx, 1 beckmann), N,
+ * U, xalpha, yalpha, eta, refract; MX_DIELECTRIC_ID 17: N, U, reflection
+ * tint, transmission tint, roughness_x, roughness_y, ior;
+ * MX_GENERALIZED_SCHLICK_ID 19: N, U, reflection tint, transmission tint,
+ * roughness_x, roughness_y, f0, f90, exponent; MX_SHEEN_ID 23: N, albedo,
+ * roughness, mode), zero-padded. d_wo (NULL: -I) is the direction
+ * MaterialX lobes are built at (it does not change what is exported). The
+ * group must have been compiled after setting the shading-system
+ * attribute "closure_export" to 1. */
+#define OSLMI_CLOSURE_PARAMS 21
+#define OSLMI_CLOSURE_RECORD (4 + OSLMI_CLOSURE_PARAMS)
+int oslmi_execute_batch_closures(oslmi_shadingsys* ss, oslmi_group* g,
+                                 const oslmi_batched_globals* bg, float* d_outputs,
+                                 const float* d_wo, int max_lobes, float* d_closures,
+                                 void* stream);
 #define OSLMI_ALIGN64 alignas(64)
 #else
 #define OSLMI_ALIGN64 _Alignas(64)

@@ -59,6 +59,7 @@ def _declare(L):
         "oslmi_batch_autotune": ([P, P, I, I, P, P], I),
         "oslmi_batch_launch_noalloc": ([P, I, P, P], I),
         "oslmi_layered_closure": ([P, I64, P, P, P, P, P, P, P, P], I),
+        "oslmi_execute_batch_closures": ([P, P, P, P, P, I, P, P], I),
         "oslmi_layered_eval_local": ([P, I64, P, P, P], I),
         "oslmi_bsdl_tables": ([P, I64], I),
         "oslmi_surrogate_create": ([I64], P),

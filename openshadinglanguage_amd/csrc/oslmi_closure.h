@@ -30,6 +30,8 @@
 #pragma once
 #include "oslmi_math.h"
 #include "oslmi_types.h"
+// after oslmi_types.h, which gives the run-time compiler its integer types
+#include "oslmi_groupargs.h"
 #ifdef OSLMI_MX_CLOSURES
 #include "oslmi_mtx.h"
 #endif
@@ -50,6 +52,11 @@ struct Lobe {
     float eta;
 #ifdef OSLMI_MX_CLOSURES
     mtx::MxLobe mx;
+#endif
+#ifdef OSLMI_CLOSURE_EXPORT
+    // the closure's parameters as testrender registers them
+    // (src/testrender/shading.cpp:185-296), for the closure export
+    float xp[OSLMI_CLOSURE_PARAMS];
 #endif
 };
 
@@ -459,5 +466,38 @@ template<int MAXL> OSLMI_DEV BsdfEval composite_eval(const LobeList<MAXL>& L, V3
         }
     return s;
 }
+
+#ifdef OSLMI_CLOSURE_EXPORT
+// testrender's closure ids (src/testrender/shading.h:25-61) of the lobe kinds
+OSLMI_DEV float export_id(int id)
+{
+    return id == kClosureDiffuse ? 3.0f                        // DIFFUSE_ID
+           : id == kClosureMicrofacetGGX || id == kClosureMicrofacetBeckmann ? 8.0f  // MICROFACET_ID
+           : id == kClosureMxDielectric ? 17.0f               // MX_DIELECTRIC_ID
+           : id == kClosureMxSchlick ? 19.0f                  // MX_GENERALIZED_SCHLICK_ID
+                                       : 23.0f;               // MX_SHEEN_ID
+}
+// Ci's lobes as process_bsdf_closure hands them to CompositeBSDF::add_bsdf:
+// out plane 0 the count (at most maxl), then per lobe k planes 1 + k R + j
+// (R = OSLMI_CLOSURE_RECORD): id, weight rgb, the parameters
+template<int MAXL>
+OSLMI_DEV void export_closure(const LobeList<MAXL>& L, float* out, int maxl, int64_t n, int64_t i)
+{
+    const int cnt = L.n < maxl ? L.n : maxl;
+    out[i]        = (float)cnt;
+#pragma unroll
+    for (int k = 0; k < MAXL; k++)
+        if (k < cnt) {
+            float* r            = out + (int64_t)(1 + k * OSLMI_CLOSURE_RECORD) * n + i;
+            r[0]                = export_id(L.l[k].id);
+            r[n]                = L.l[k].weight.x;
+            r[2 * n]            = L.l[k].weight.y;
+            r[3 * n]            = L.l[k].weight.z;
+#pragma unroll
+            for (int j = 0; j < OSLMI_CLOSURE_PARAMS; j++)
+                r[(4 + j) * n] = L.l[k].xp[j];
+        }
+}
+#endif
 
 }  // namespace oslmi

@@ -1498,7 +1498,7 @@ struct Emitter {
             return false;
         std::ostringstream k;
         k << "// Emitted by the oslmi group emitter for group '" << g.name << "'.\n";
-        k << prelude(uses_mx);
+        k << prelude(uses_mx, g.can_export);
         k << "extern \"C\" __global__ " << kernel_attributes(ss) << " void oslmi_group_main(GroupArgs A)\n{\n";
         k << "    using Closure = LobeList<" << closure_capacity() << ">;\n";
         if (uses_cs)
@@ -1516,7 +1516,7 @@ struct Emitter {
         return true;
     }
 
-    static std::string prelude(bool mx)
+    static std::string prelude(bool mx, bool xp = false)
     {
         return std::string(mx ? "#define OSLMI_MX_CLOSURES 1\n" : "")
                + "#include \"oslmi_rt.h\"\n#include \"oslmi_groupargs.h\"\n"
@@ -1541,7 +1541,10 @@ bool emit_group(const ShadingSys& ss, Group& g, std::string& src, std::string& e
     return true;
 }
 
+        if (g.surface_closure && ss.closure_export)
+            k << "        if (A.clo) export_closure(ci_, A.clo, A.clo_max, A.n, i);\n";
 // One persistent kernel for a list of groups. Work arrives as tiles
+        g.can_export    = g.surface_closure && ss.closure_export;
 // {item, begin, end} (BatchTile), sorted by group by the host; every
 // workgroup pulls the next tile from a device counter and runs the tile's
 // group body over blockDim.x * ppt consecutive points (ppt strided passes,
@@ -1578,6 +1581,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
         c << "        const GroupArgs& A = args[" << gi << "];\n";
         c << "        using Closure = LobeList<" << e.closure_capacity() << ">;\n";
         if (e.uses_cs)
+               + std::string(xp ? "#define OSLMI_CLOSURE_EXPORT 1\n" : "")
             c << "        const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
         c << "        for (;;) {\n";
         c << "#pragma unroll 1\n";

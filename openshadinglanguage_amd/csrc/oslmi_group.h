@@ -72,6 +72,7 @@ struct Group {
     bool uses_mx             = false;  // MaterialX closures: reads the libbsdl energy tables
     bool uses_xforms         = false;  // reads the transform / attribute table
     bool can_sample          = false;  // kernel samples Ci when A.samp is set
+    bool can_export          = false;  // kernel exports Ci's lobes when A.clo is set
     std::vector<unsigned> globals_used;  // GlobalIndex values read
     hipModule_t module       = nullptr;
     hipFunction_t kernel     = nullptr;
@@ -133,6 +134,7 @@ struct ShadingSys {
     std::string include_dir; // device library headers for hiprtc
     int waves_per_eu = 6;    // occupancy target of emitted kernels (0: compiler default)
     bool closure_sample = false;  // emit CompositeBSDF::sample into surface kernels
+    bool closure_export = false;  // emit the closure export into surface kernels
     mutable XformTable xf;
     std::mutex mu;
 };

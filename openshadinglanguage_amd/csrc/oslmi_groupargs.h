@@ -36,7 +36,18 @@ struct GroupArgs {
     float* samp;
     // libbsdl energy tables (kernels with MaterialX closures)
     const float* bsdl;
+    // optional closure export (kernels emitted with export): 1 +
+    // clo_max * OSLMI_CLOSURE_RECORD planes (count, then per lobe id,
+    // weight rgb, parameters)
+    float* clo;
+    int32_t clo_max;
+    int32_t clo_pad;
 };
+
+// planes per exported lobe: id, weight rgb, 21 parameter floats (the
+// largest set, generalized_schlick_bsdf's)
+#define OSLMI_CLOSURE_PARAMS 21
+#define OSLMI_CLOSURE_RECORD (4 + OSLMI_CLOSURE_PARAMS)
 
 // One unit of work of the persistent batch kernel: points [begin, end) of
 // batch item `item` (which is also the group's case in the kernel).

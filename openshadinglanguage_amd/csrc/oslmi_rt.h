@@ -195,6 +195,17 @@ OSLMI_DEV Dv triple(const Df& a, const Df& b, const Df& c) { return make_dv(a, b
 // group (the number of closure constructors it can reach, at most 8);
 // primitives make one-lobe lists, ADD concatenates, MUL scales.
 typedef LobeList<8> Closure;
+#ifdef OSLMI_CLOSURE_EXPORT
+// the exported parameters of a lobe, in testrender's registration order
+// (src/testrender/shading.cpp:185-296; MTX/bsdf_*_decl.h entry()),
+// zero-padded
+OSLMI_DEV void set_xp(Lobe& l, const float* v, int k)
+{
+#pragma unroll
+    for (int j = 0; j < OSLMI_CLOSURE_PARAMS; j++)
+        l.xp[j] = j < k ? v[j] : 0.0f;
+}
+#endif
 #ifdef OSLMI_MX_CLOSURES
 // a component (not the null closure); as the top of a layer, a lobe other
 // than a MaterialX one is opaque (evaluate_layer_opacity's default case)
@@ -209,6 +220,12 @@ OSLMI_DEV LobeList<1> closure_diffuse(V3 N)
     LobeList<1> c;
     Lobe l;
     l.id = kClosureDiffuse; l.weight = V3(1.0f); l.N = N; l.alpha = 0.0f; l.eta = 0.0f;
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        const float v[3] = { N.x, N.y, N.z };  // DiffuseParams: N
+        set_xp(l, v, 3);
+    }
+#endif
     c.add(l);
 #ifdef OSLMI_MX_CLOSURES
     mark_component(c);
@@ -220,6 +237,14 @@ OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
     LobeList<1> c;
     Lobe l;
     l.id = kClosureMicrofacetGGX; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        // MicrofacetParams: dist (0 ggx, 1 beckmann), N, U, xalpha, yalpha,
+        // eta, refract
+        const float v[11] = { 0.0f, N.x, N.y, N.z, 0.0f, 0.0f, 0.0f, alpha, alpha, eta, 0.0f };
+        set_xp(l, v, 11);
+    }
+#endif
     c.add(l);
 #ifdef OSLMI_MX_CLOSURES
     mark_component(c);
@@ -231,6 +256,14 @@ OSLMI_DEV LobeList<1> closure_microfacet_beckmann(V3 N, float alpha, float eta)
     LobeList<1> c;
     Lobe l;
     l.id = kClosureMicrofacetBeckmann; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        // MicrofacetParams: dist (0 ggx, 1 beckmann), N, U, xalpha, yalpha,
+        // eta, refract
+        const float v[11] = { 1.0f, N.x, N.y, N.z, 0.0f, 0.0f, 0.0f, alpha, alpha, eta, 0.0f };
+        set_xp(l, v, 11);
+    }
+#endif
     c.add(l);
 #ifdef OSLMI_MX_CLOSURES
     mark_component(c);
@@ -315,6 +348,15 @@ OSLMI_DEV LobeList<1> closure_mx_dielectric(V3 N, V3 U, V3 refl_tint, float rx, 
     LobeList<1> c = closure_mx_lobe(kClosureMxDielectric, N);
     mtx::mx_micro(&c.l[0].mx, mtx::kMxDielectric, N, U, refl_tint, rx, ry, ior, V3(0.0f), V3(0.0f),
                   0.0f, wo, T);
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        // dielectric_bsdf: N, U, reflection tint, transmission tint (0),
+        // roughness_x, roughness_y, ior
+        const float v[15] = { N.x, N.y, N.z, U.x, U.y, U.z, refl_tint.x, refl_tint.y, refl_tint.z,
+                              0.0f, 0.0f, 0.0f, rx, ry, ior };
+        set_xp(c.l[0], v, 15);
+    }
+#endif
     c.op = V3(1.0f) * (V3(1.0f) - V3(mtx::mx_filter_o(c.l[0].mx)));
     return c;
 }
@@ -324,6 +366,16 @@ OSLMI_DEV LobeList<1> closure_mx_schlick(V3 N, V3 U, V3 refl_tint, float rx, flo
     LobeList<1> c = closure_mx_lobe(kClosureMxSchlick, N);
     mtx::mx_micro(&c.l[0].mx, mtx::kMxSchlick, N, U, refl_tint, rx, ry, 0.0f, F0, F90, exponent, wo,
                   T);
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        // generalized_schlick_bsdf: N, U, reflection tint, transmission
+        // tint (0), roughness_x, roughness_y, f0, f90, exponent
+        const float v[21] = { N.x,  N.y,  N.z,  U.x,  U.y,   U.z,   refl_tint.x, refl_tint.y,
+                              refl_tint.z, 0.0f, 0.0f, 0.0f, rx, ry, F0.x, F0.y, F0.z, F90.x,
+                              F90.y, F90.z, exponent };
+        set_xp(c.l[0], v, 21);
+    }
+#endif
     c.op = V3(1.0f) * (V3(1.0f) - V3(mtx::mx_filter_o(c.l[0].mx)));
     return c;
 }
@@ -331,6 +383,13 @@ OSLMI_DEV LobeList<1> closure_mx_sheen(V3 N, V3 albedo, float roughness, V3 wo)
 {
     LobeList<1> c = closure_mx_lobe(kClosureMxSheen, N);
     mtx::mx_sheen(&c.l[0].mx, N, albedo, roughness, wo);
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        // sheen_bsdf: N, albedo, roughness, mode (0)
+        const float v[8] = { N.x, N.y, N.z, albedo.x, albedo.y, albedo.z, roughness, 0.0f };
+        set_xp(c.l[0], v, 8);
+    }
+#endif
     c.op = V3(1.0f) * (V3(1.0f) - V3(mtx::mx_filter_o(c.l[0].mx)));
     return c;
 }

@@ -426,6 +426,13 @@ int oslmi_shadingsys_attribute(oslmi_shadingsys* s, const char* name, const char
         if (v != "0" && v != "1")
             return set_error(OSLMI_ERR_INVALID, "closure_sample must be 0 or 1");
         s->ss.closure_sample = v == "1";
+    } else if (n == "closure_export") {
+        // surface kernels JIT-compiled from now on also export Ci's lobes
+        // (oslmi_execute_batch_closures)
+        std::string v(value);
+        if (v != "0" && v != "1")
+            return set_error(OSLMI_ERR_INVALID, "closure_export must be 0 or 1");
+        s->ss.closure_export = v == "1";
     } else if (n == "gpu_waves_per_eu") {
         // occupancy target (waves per SIMD) the emitted kernels are
         // register-allocated for; 0 leaves it to the compiler
@@ -694,6 +701,40 @@ int oslmi_execute_batch_sample(oslmi_shadingsys* s, oslmi_group* gh,
     A.wo   = d_wo;
     A.rnd  = d_rnd;
     A.samp = d_samples;
+    if (bg->n == 0)
+        return OSLMI_OK;
+    void* args[] = { &A };
+    return check_hip(hipModuleLaunchKernel(g.kernel, grid_for(bg->n), 1, 1, 256, 1, 1, 0,
+                                           (hipStream_t)stream, args, nullptr),
+                     "hipModuleLaunchKernel");
+}
+
+int oslmi_execute_batch_closures(oslmi_shadingsys* s, oslmi_group* gh,
+                                 const oslmi_batched_globals* bg, float* d_outputs,
+                                 const float* d_wo, int max_lobes, float* d_closures,
+                                 void* stream)
+{
+    if (!s || !gh || !bg || !d_closures)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch_closures: null argument");
+    if (max_lobes < 1 || max_lobes > 8)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_execute_batch_closures: max_lobes must be 1..8");
+    Group& g = *gh->g;
+    if (!g.jitted) {
+        if (int rc = oslmi_jit_group(s, gh))
+            return rc;
+    }
+    if (!g.can_export)
+        return set_error(OSLMI_ERR_INVALID,
+                         "oslmi_execute_batch_closures: group " + g.name
+                             + " has no surface closure or was compiled without the"
+                               " closure_export attribute");
+    GroupArgs A;
+    if (int rc = make_group_args(s->ss, g, bg, d_outputs, nullptr, nullptr, nullptr,
+                                 "oslmi_execute_batch_closures", A))
+        return rc;
+    A.wo      = d_wo;
+    A.clo     = d_closures;
+    A.clo_max = max_lobes;
     if (bg->n == 0)
         return OSLMI_OK;
     void* args[] = { &A };

@@ -225,6 +225,26 @@ class ShadingSystem:
                                                _ptr(samples), sp))
 
 
+    def closures(self, group, n, globals_, closures, max_lobes=8, wo=None, outputs=None,
+                 stream=None):
+        """Batched execution with the surface closure handed out as
+        testrender's list of (closure id, weight rgb, parameters)
+        (oslmi_execute_batch_closures): closures 1 + max_lobes *
+        CLOSURE_RECORD planes. Needs the group compiled after
+        attribute("closure_export", 1)."""
+        bg = batched_globals(n, globals_)
+        import torch
+        sp = ctypes.c_void_p(stream.cuda_stream if stream is not None
+                             else torch.cuda.current_stream().cuda_stream)
+        check(lib().oslmi_execute_batch_closures(self.h, group.h, ctypes.byref(bg),
+                                                 _ptr(outputs), _ptr(wo), max_lobes,
+                                                 _ptr(closures), sp))
+
+
+# planes per exported closure component: id, weight rgb, 21 parameters
+CLOSURE_RECORD = 25
+
+
 def build_fbm_cell(ss=None, outputs=("Cout",)):
     """The single-layer fbm_cell group."""
     ss = ss or ShadingSystem()

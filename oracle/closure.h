@@ -14,7 +14,8 @@ struct Lobe {
     V3 N;
     float alpha;  // microfacet xalpha = yalpha
     float eta;
-    mtx::MxLobe mx;  // kLobeMx: a MaterialX lobe built at the shading point's wo
+    mtx::MxLobe mx;    // kLobeMx: a MaterialX lobe built at the shading point's wo
+    mtx::MxParams mxp;  // kLobeMx: its closure parameters
 };
 
 struct BsdfSample {

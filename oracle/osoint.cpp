@@ -990,6 +990,7 @@ struct Exec {
                     l.kind   = kLobeMx;
                     l.weight = cw;
                     l.N      = nd.mx.N;
+                    l.mxp    = nd.mx;
                     mtx::mx_build(&l.mx, nd.mx, wo);
                     c.add(l);
                 } else {
@@ -1191,6 +1192,81 @@ void oref_group_run(void* h, const float* P, const float* N, const float* wo, co
                     int64_t n, float* out4, float* eval4, int nthreads)
 {
     run_group((Group*)h, P, N, nullptr, nullptr, wo, wi, nullptr, n, out4, eval4, nullptr, nthreads);
+}
+
+// Ci's lobes as process_bsdf_closure hands them to CompositeBSDF::add_bsdf
+// (MaterialX lobes built at wo), in the closure-export layout of the C ABI
+// (oslmi_execute_batch_closures): out plane 0 the count (at most maxl),
+// then per lobe k planes 1 + 25 k + j: testrender's closure id, weight rgb,
+// 21 parameter floats in testrender's registration order, zero-padded.
+void oref_group_closures(void* h, const float* P, const float* N, const float* dPdu,
+                         const float* dPdv, const float* wo, int64_t n, int maxl, float* out)
+{
+    Group* g = (Group*)h;
+    Exec ex(*g);
+    const int R = 25;
+    for (int64_t i = 0; i < n; ++i) {
+        ex.P    = V3(P[i], P[n + i], P[2 * n + i]);
+        ex.N    = V3(N[i], N[n + i], N[2 * n + i]);
+        ex.dPdu = dPdu ? V3(dPdu[i], dPdu[n + i], dPdu[2 * n + i]) : V3(0.0f);
+        ex.dPdv = dPdv ? V3(dPdv[i], dPdv[n + i], dPdv[2 * n + i]) : V3(0.0f);
+        ex.run();
+        const Composite c = ex.process(V3(wo[i], wo[n + i], wo[2 * n + i]));
+        const int cnt     = c.n < maxl ? c.n : maxl;
+        out[i]            = (float)cnt;
+        for (int k = 0; k < cnt; ++k) {
+            const Lobe& l = c.lobes[k];
+            float v[R]    = { 0 };
+            v[1]          = l.weight.x;
+            v[2]          = l.weight.y;
+            v[3]          = l.weight.z;
+            float* p      = v + 4;
+            auto put3     = [&](V3 a) {
+                *p++ = a.x;
+                *p++ = a.y;
+                *p++ = a.z;
+            };
+            if (l.kind == kLobeDiffuse) {
+                v[0] = 3;  // DIFFUSE_ID
+                put3(l.N);
+            } else if (l.kind == kLobeGGXRefl || l.kind == kLobeBeckmannRefl) {
+                v[0] = 8;  // MICROFACET_ID: dist, N, U, xalpha, yalpha, eta, refract
+                *p++ = l.kind == kLobeBeckmannRefl ? 1.0f : 0.0f;
+                put3(l.N);
+                put3(V3(0.0f));
+                *p++ = l.alpha;
+                *p++ = l.alpha;
+                *p++ = l.eta;
+                *p++ = 0.0f;
+            } else {
+                const mtx::MxParams& m = l.mxp;
+                if (m.kind == 2) {
+                    v[0] = 23;  // MX_SHEEN_ID: N, albedo, roughness, mode
+                    put3(m.N);
+                    put3(m.tint);
+                    *p++ = m.rx;
+                    *p++ = 0.0f;
+                } else {
+                    v[0] = m.kind == 0 ? 17 : 19;  // MX_DIELECTRIC_ID, MX_GENERALIZED_SCHLICK_ID
+                    put3(m.N);
+                    put3(m.U);
+                    put3(m.tint);
+                    put3(V3(0.0f));
+                    *p++ = m.rx;
+                    *p++ = m.ry;
+                    if (m.kind == 0) {
+                        *p++ = m.ior;
+                    } else {
+                        put3(m.F0);
+                        put3(m.F90);
+                        *p++ = m.exponent;
+                    }
+                }
+            }
+            for (int j = 0; j < R; ++j)
+                out[(int64_t)(1 + k * R + j) * n + i] = v[j];
+        }
+    }
 }
 
 // Both, with the globals dPdu and dPdv (3 planes each, or null: zero) and

@@ -482,3 +482,17 @@ def group_run_sample(group, P, N, wo, rnd, nthreads=1):
     samp = np.zeros((7, n), np.float32)
     L.oref_group_run_sample(group.h, *[_p(x) for x in a], n, _p(out), _p(samp), nthreads)
     return out, samp
+def group_closures(group, P, N, wo, dPdu=None, dPdv=None, max_lobes=8):
+    """Ci's lobes in the closure-export layout: [1 + 25 max_lobes, n]."""
+    L = group.L
+    L.oref_group_closures.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int64, ctypes.c_int,
+                                                              ctypes.c_void_p]
+    f = lambda x: None if x is None else np.ascontiguousarray(x, np.float32)  # noqa: E731
+    P, N, wo, dPdu, dPdv = map(f, (P, N, wo, dPdu, dPdv))
+    n = P.shape[1]
+    out = np.zeros((1 + 25 * max_lobes, n), np.float32)
+    q = lambda x: None if x is None else _p(x)  # noqa: E731
+    L.oref_group_closures(group.h, _p(P), _p(N), q(dPdu), q(dPdv), _p(wo), n, max_lobes, _p(out))
+    return out
+
+

@@ -2,11 +2,12 @@
 // lowering: the kernel source (KERNEL, a file name) is compiled as plain
 // C++ with the device qualifiers defined away, and its grid-stride loop
 // runs on one host thread over every point (blockIdx = threadIdx = 0,
-// blockDim = gridDim = 1). Evaluation only: reads n, then 3-plane P, N,
-// dPdu, dPdv, wo, wi (float32), and the libbsdl table file; writes the
-// 4 eval planes.
+// blockDim = gridDim = 1). Reads n, then 3-plane P, N, dPdu, dPdv, wo, wi
+// (float32), and the libbsdl table file; writes the 4 eval planes, or with
+// a fourth argument "export" the closure export (1 + 8 records of
+// OSLMI_CLOSURE_RECORD planes) of a kernel built with it.
 //
-// usage: harness <input> <output> <bsdl_luts.bin>
+// usage: harness <input> <output> <bsdl_luts.bin> [export]
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -67,8 +68,9 @@ static inline void __syncthreads() {}
 
 int main(int argc, char** argv)
 {
-    if (argc != 4)
+    if (argc != 4 && argc != 5)
         return 1;
+    const bool xp = argc == 5;
     FILE* f = fopen(argv[1], "rb");
     int64_t n = 0;
     if (!f || fread(&n, 8, 1, f) != 1 || n < 1)
@@ -92,12 +94,20 @@ int main(int argc, char** argv)
     A.g[oslmi::kG_dPdu] = du.data();
     A.g[oslmi::kG_dPdv] = dv.data();
     A.wo                = wo.data();
-    A.wi                = wi.data();
-    A.eval              = ev.data();
     A.bsdl              = lut.data();
+    std::vector<float> clo;
+    if (xp) {
+        clo.assign((size_t)(1 + 8 * OSLMI_CLOSURE_RECORD) * n, 0.0f);
+        A.clo     = clo.data();
+        A.clo_max = 8;
+    } else {
+        A.wi   = wi.data();
+        A.eval = ev.data();
+    }
     oslmi_group_main(A);
     FILE* o = fopen(argv[2], "wb");
-    if (!o || fwrite(ev.data(), 4, 4 * n, o) != (size_t)(4 * n))
+    const std::vector<float>& res = xp ? clo : ev;
+    if (!o || fwrite(res.data(), 4, res.size(), o) != res.size())
         return 5;
     fclose(o);
     return 0;

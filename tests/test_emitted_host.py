@@ -22,18 +22,17 @@ CSRC = os.path.join(HERE, "..", "openshadinglanguage_amd", "csrc")
 LUTS = os.path.join(HERE, "..", "oracle", "build", "bsdl_luts.bin")
 
 
-@pytest.mark.parametrize("k", range(len(X.programs())), ids=[p[0] for p in X.programs()])
-def test_host_built_kernel_matches_interpreter(tmp_path, k):
-    name, o = X.programs()[k]
+def _host_run(tmp_path, name, text, inp, export):
     ss = S.ShadingSystem()
-    g = S.build_generated_group(ss, ([(name, "surface", o.text())], []), name, outputs=())
+    if export:
+        ss.attribute("closure_export", 1)
+    g = S.build_generated_group(ss, ([(name, "surface", text)], []), name, outputs=())
     src = tmp_path / (name + ".hip")
     src.write_text(ss.emit(g))
     exe = tmp_path / "harness"
     subprocess.check_call(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-I", CSRC,
                            "-I", str(tmp_path), '-DKERNEL="%s.hip"' % name, "-o", str(exe),
                            os.path.join(HERE, "hostkernel", "harness.cpp")])
-    inp = X.inputs(1500, 60 + k)
     n = inp["P"].shape[1]
     fin = tmp_path / "in.bin"
     with open(fin, "wb") as f:
@@ -42,9 +41,29 @@ def test_host_built_kernel_matches_interpreter(tmp_path, k):
             f.write(np.ascontiguousarray(inp[key], np.float32).tobytes())
     fout = tmp_path / "out.bin"
     O.mtx_tables()  # the oracle's table file exists once the oracle has loaded it
-    subprocess.check_call([str(exe), str(fin), str(fout), LUTS])
-    ev = np.fromfile(fout, np.float32).reshape(4, n)
+    subprocess.check_call([str(exe), str(fin), str(fout), LUTS] + (["export"] if export else []))
+    return np.fromfile(fout, np.float32).reshape(-1, n)
+
+
+@pytest.mark.parametrize("k", range(len(X.programs())), ids=[p[0] for p in X.programs()])
+def test_host_built_kernel_matches_interpreter(tmp_path, k):
+    name, o = X.programs()[k]
+    inp = X.inputs(1500, 60 + k)
+    ev = _host_run(tmp_path, name, o.text(), inp, export=False)
     gr = O.Group([(name, "surface", o.text())], [])
     _, rev, _ = O.group_run_ex(gr, inp["P"], inp["N"], inp["wo"], dPdu=inp["dPdu"],
                                dPdv=inp["dPdv"], wi=inp["wi"])
     assert np.array_equal(ev, rev), (name, np.mean(ev == rev))
+
+
+@pytest.mark.parametrize("k", range(len(X.programs())), ids=[p[0] for p in X.programs()])
+def test_host_built_closure_export_matches_interpreter(tmp_path, k):
+    """The exported lobe lists (testrender ids, weights, parameters) equal
+    the interpreter's walk of the closure tree."""
+    name, o = X.programs()[k]
+    inp = X.inputs(800, 70 + k)
+    got = _host_run(tmp_path, name, o.text(), inp, export=True)
+    gr = O.Group([(name, "surface", o.text())], [])
+    ref = O.group_closures(gr, inp["P"], inp["N"], inp["wo"], dPdu=inp["dPdu"], dPdv=inp["dPdv"])
+    assert np.array_equal(got, ref), (name, np.argwhere(got != ref)[:4].tolist())
+    assert (ref[0] >= 1).all()
