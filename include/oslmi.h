@@ -49,6 +49,18 @@ int oslmi_init(void);
 int oslmi_noise(const char* name, int outdim, int indim, int derivs, const float* d_in,
                 float* d_out, int64_t n, void* stream);
 
+/* Periodic noise (pnoise/psnoise with or without a name): the same planes
+ * plus d_period, indim planes of n floats holding each point's period per
+ * input dimension (floats without derivatives, as in the reference).
+ * Replaces the osl_b16_<ISA>_{pnoise,psnoise,pcellnoise,phashnoise}_<args>
+ * shadeops (wide_opnoise_periodic_impl_xmacro.h; PeriodicNoise,
+ * PeriodicSNoise, PeriodicAdaptionOf at src/include/OSL/oslnoise.h:486-548,
+ * :2531-2760). name: "perlin"|"snoise"|"psnoise", "uperlin"|"noise"|"pnoise",
+ * "cell"|"cellnoise", "hash"|"hashnoise" (the periodic branch of
+ * batched_llvm_gen.cpp:6096-6134; simplex has no periodic form). */
+int oslmi_pnoise(const char* name, int outdim, int indim, int derivs, const float* d_in,
+                 const float* d_period, float* d_out, int64_t n, void* stream);
+
 /* hash() shadeop over 1..4 float planes (src/liboslexec/wide/wide_ophash.cpp:28-80). */
 int oslmi_hash(int indim, const float* d_in, int32_t* d_out, int64_t n, void* stream);
 
@@ -162,18 +174,6 @@ int oslmi_execute_batch_sample(oslmi_shadingsys* ss, oslmi_group* g,
                                const oslmi_batched_globals* bg, float* d_outputs,
                                const float* d_wo, const float* d_rnd, float* d_samples,
                                void* stream);
-
-/* ---- The reference's own batch layout ----------------------------------
- * BatchedShaderGlobals<16> byte for byte (src/include/OSL/
- * batched_shaderglobals.h:21-140): the 64-byte UniformShaderGlobals block,
- * then VaryingShaderGlobals<16>, whose Block<Vec3,16> fields are x[16],
- * y[16], z[16] (src/include/OSL/wide.h:439-445) and Block<float,16> /
- * Block<int,16> / Block<pointer,16> 16 values each. 4032 bytes, 64-byte
- * aligned. A renderer that fills these for the batched CPU path hands the
- * same blocks (copied to device memory) to oslmi_execute_wide16. The
- * uniform pointers and object2common / shader2common / Ci are opaque
- * renderer data here and are not read. */
-#if defined(__cplusplus)
 /* The same execution with the surface closure Ci handed out as testrender
  * gets it from OSL: the list of components process_bsdf_closure passes to
  * CompositeBSDF::add_bsdf (src/testrender/shading.cpp:1448-1667), each as
@@ -199,6 +199,18 @@ int oslmi_execute_batch_closures(oslmi_shadingsys* ss, oslmi_group* g,
                                  const oslmi_batched_globals* bg, float* d_outputs,
                                  const float* d_wo, int max_lobes, float* d_closures,
                                  void* stream);
+
+/* ---- The reference's own batch layout ----------------------------------
+ * BatchedShaderGlobals<16> byte for byte (src/include/OSL/
+ * batched_shaderglobals.h:21-140): the 64-byte UniformShaderGlobals block,
+ * then VaryingShaderGlobals<16>, whose Block<Vec3,16> fields are x[16],
+ * y[16], z[16] (src/include/OSL/wide.h:439-445) and Block<float,16> /
+ * Block<int,16> / Block<pointer,16> 16 values each. 4032 bytes, 64-byte
+ * aligned. A renderer that fills these for the batched CPU path hands the
+ * same blocks (copied to device memory) to oslmi_execute_wide16. The
+ * uniform pointers and object2common / shader2common / Ci are opaque
+ * renderer data here and are not read. */
+#if defined(__cplusplus)
 #define OSLMI_ALIGN64 alignas(64)
 #else
 #define OSLMI_ALIGN64 _Alignas(64)

@@ -23,6 +23,7 @@ def _declare(L):
         "oslmi_last_error": ([], C),
         "oslmi_init": ([], I),
         "oslmi_noise": ([C, I, I, I, P, P, I64, P], I),
+        "oslmi_pnoise": ([C, I, I, I, P, P, P, I64, P], I),
         "oslmi_hash": ([I, P, P, I64, P], I),
         "oslmi_shade_fbm_cell": ([I64, P, P, P], I),
         "oslmi_shade_layered3": ([I64, P, P, P, P, P, P, P, P, P], I),
@@ -58,8 +59,8 @@ def _declare(L):
         "oslmi_batch_capture": ([P, P], I),
         "oslmi_batch_autotune": ([P, P, I, I, P, P], I),
         "oslmi_batch_launch_noalloc": ([P, I, P, P], I),
-        "oslmi_layered_closure": ([P, I64, P, P, P, P, P, P, P, P], I),
         "oslmi_execute_batch_closures": ([P, P, P, P, P, I, P, P], I),
+        "oslmi_layered_closure": ([P, I64, P, P, P, P, P, P, P, P], I),
         "oslmi_layered_eval_local": ([P, I64, P, P, P], I),
         "oslmi_bsdl_tables": ([P, I64], I),
         "oslmi_surrogate_create": ([I64], P),

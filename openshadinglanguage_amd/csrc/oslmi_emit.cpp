@@ -215,24 +215,33 @@ struct Emitter {
         return o == "noise" || o == "snoise" || o == "pnoise" || o == "psnoise" || o == "cellnoise"
                || o == "hashnoise";
     }
-    // index of the first coordinate arg and the noise name for a noise op
-    void noise_layout(int L, const OOp& op, std::string& name, int& first, int& ncoord)
+    static bool is_periodic_noise(const std::string& o) { return o == "pnoise" || o == "psnoise"; }
+    // The noise name, the index of the first coordinate arg and the number
+    // of coordinate args of a noise op; nnum counts the numeric args
+    // (coordinates, then for pnoise/psnoise as many periods) before the
+    // optional (name, value) pairs. The periodic forms take (S, Sper) or
+    // (S, T, Sper, Tper) (batched_llvm_gen.cpp:6066-6081).
+    void noise_layout(int L, const OOp& op, std::string& name, int& first, int& ncoord,
+                      int* nnum = nullptr)
     {
         first = 1;
-        name  = op.name == "snoise" ? "perlin"
-                : op.name == "cellnoise" ? "cell"
-                : op.name == "hashnoise" ? "hash"
-                : "uperlin";
+        name  = (op.name == "snoise" || op.name == "psnoise") ? "perlin"
+                : op.name == "cellnoise"                      ? "cell"
+                : op.name == "hashnoise"                      ? "hash"
+                                                              : "uperlin";
         if (op.args.size() > 1 && S(L, op.args[1]).type.base == kString) {
             strval(L, op.args[1], name);
             first = 2;
         }
-        ncoord = 0;
+        int n = 0;
         for (size_t a = first; a < op.args.size(); ++a) {
             if (S(L, op.args[a]).type.base == kString)
                 break;
-            ++ncoord;
+            ++n;
         }
+        if (nnum)
+            *nnum = n;
+        ncoord = is_periodic_noise(op.name) ? n / 2 : n;
     }
 
     bool differentiable(const std::string& o)
@@ -605,8 +614,13 @@ struct Emitter {
     void emit_noise(int L, const OOp& op)
     {
         std::string nm;
-        int first, nc;
-        noise_layout(L, op, nm, first, nc);
+        int first, nc, nnum;
+        noise_layout(L, op, nm, first, nc, &nnum);
+        const bool periodic = is_periodic_noise(op.name);
+        if (periodic && (nnum != 2 && nnum != 4)) {
+            fail(op.name + " needs (P, Pperiod) or (P, t, Pperiod, tperiod)");
+            return;
+        }
         int r   = op.args[0];
         Kind rk = kind(L, r);
         bool rd = rk == K_DF || rk == K_DV;
@@ -634,6 +648,23 @@ struct Emitter {
                 dim += 1;
             }
         }
+        // periods: plain floats, one per coordinate component
+        std::vector<std::string> pers;
+        for (int a = nc; periodic && a < nnum; ++a) {
+            int s = op.args[first + a];
+            if (S(L, s).type.base == kTriple) {
+                std::string v = ex(L, s, K_V3);
+                pers.push_back("(" + v + ").x");
+                pers.push_back("(" + v + ").y");
+                pers.push_back("(" + v + ").z");
+            } else {
+                pers.push_back(ex(L, s, K_FLOAT));
+            }
+        }
+        if (periodic && (int)pers.size() != dim) {
+            fail(op.name + ": the periods do not match the coordinates");
+            return;
+        }
         std::string T = rd ? "Df" : "float";
         std::string out;
         auto join = [&](int n) {
@@ -642,27 +673,36 @@ struct Emitter {
                 s += (i ? ", " : "") + comps[i];
             return s;
         };
+        // the lattice map argument of the periodic perlin
+        const std::string wa = periodic ? ", w_" : "";
         if (nm == "perlin" || nm == "snoise" || nm == "uperlin" || nm == "noise") {
             bool uns = (nm == "uperlin" || nm == "noise");
             std::string body;
+            std::string wdecl;
+            if (periodic) {
+                std::string pa;
+                for (int i = 0; i < dim; ++i)
+                    pa += (i ? ", " : "") + pers[i];
+                wdecl = "const Wrap w_ = make_wrap(" + pa + "); ";
+            }
             if (!tr) {
-                std::string call = dim == 1   ? "perlin1<" + T + ">(" + join(1) + ", 0)"
-                                   : dim == 2 ? "perlin2<" + T + ">(" + join(2) + ", 0)"
-                                   : dim == 3 ? "perlin3<" + T + ">(" + join(3) + ")"
-                                              : "perlin4<" + T + ">(" + join(4) + ", 0)";
+                std::string call = dim == 1   ? "perlin1<" + T + ">(" + join(1) + ", 0" + wa + ")"
+                                   : dim == 2 ? "perlin2<" + T + ">(" + join(2) + ", 0" + wa + ")"
+                                   : dim == 3 ? "perlin3<" + T + ">(" + join(3) + wa + ")"
+                                              : "perlin4<" + T + ">(" + join(4) + ", 0" + wa + ")";
                 body = uns ? "0.5f * (" + call + " + 1.0f)" : call;
-                line("{ " + var(L, r) + " = " + body + "; }");
+                line("{ " + wdecl + var(L, r) + " = " + body + "; }");
             } else {
                 std::string c[3];
                 if (dim == 3) {
-                    line("{ " + T + " c_[3]; vperlin3<" + T + ">(" + join(3) + ", c_);");
+                    line("{ " + wdecl + T + " c_[3]; vperlin3<" + T + ">(" + join(3) + ", c_" + wa + ");");
                 } else {
-                    line("{ " + T + " c_[3];");
+                    line("{ " + wdecl + T + " c_[3];");
                     for (int k = 0; k < 3; ++k) {
                         std::string sh = std::to_string(8 * k);
-                        std::string call = dim == 1   ? "perlin1<" + T + ">(" + join(1) + ", " + sh + ")"
-                                           : dim == 2 ? "perlin2<" + T + ">(" + join(2) + ", " + sh + ")"
-                                                      : "perlin4<" + T + ">(" + join(4) + ", " + sh + ")";
+                        std::string call = dim == 1   ? "perlin1<" + T + ">(" + join(1) + ", " + sh + wa + ")"
+                                           : dim == 2 ? "perlin2<" + T + ">(" + join(2) + ", " + sh + wa + ")"
+                                                      : "perlin4<" + T + ">(" + join(4) + ", " + sh + wa + ")";
                         line("  c_[" + std::to_string(k) + "] = " + call + ";");
                     }
                 }
@@ -675,16 +715,22 @@ struct Emitter {
         }
         if (nm == "cell" || nm == "cellnoise" || nm == "hash" || nm == "hashnoise") {
             std::string cl = (nm == "cell" || nm == "cellnoise") ? "true" : "false";
-            // values only
+            // values only; the periodic forms hash the wrapped coordinates
             std::vector<std::string> vc;
-            for (auto& c : comps)
-                vc.push_back(rd ? "(" + c + ").v" : c);
+            for (size_t i = 0; i < comps.size(); ++i) {
+                std::string c = rd ? "(" + comps[i] + ").v" : comps[i];
+                vc.push_back(periodic ? "wrap_period(" + c + ", " + pers[i] + ")" : c);
+            }
             std::string args;
             for (int i = 0; i < dim; ++i)
                 args += (i ? ", " : "") + vc[i];
             std::string fn = (tr ? "vintnoise" : "intnoise") + std::to_string(dim) + "<" + cl + ">(" + args + ")";
             std::string v  = rd ? (tr ? "Dv(" + fn + ")" : "Df(" + fn + ")") : fn;
             line("{ " + var(L, r) + " = " + v + "; }");
+            return;
+        }
+        if ((nm == "simplex" || nm == "usimplex") && periodic) {
+            fail("periodic noise type '" + nm + "' is unknown");
             return;
         }
         if (nm == "simplex" || nm == "usimplex") {
@@ -735,7 +781,7 @@ struct Emitter {
         }
         if (nm == "gabor") {
             GaborOptions go;
-            for (size_t a = first + nc; a + 1 < op.args.size(); a += 2) {
+            for (size_t a = first + nnum; a + 1 < op.args.size(); a += 2) {
                 std::string key;
                 strval(L, op.args[a], key);
                 std::vector<float> v;
@@ -751,10 +797,6 @@ struct Emitter {
                     go.direction[1] = v[1];
                     go.direction[2] = v[2];
                 }
-            }
-            if (tr) {
-                fail("vector gabor noise is not supported");
-                return;
             }
             GaborUniform gu = build_gabor_uniform(go);
             std::string gus = "GaborUniform{" + flit(gu.a) + "," + flit(gu.lambda) + ","
@@ -776,10 +818,27 @@ struct Emitter {
             }
             while (dc.size() < 3)
                 dc.push_back("Df(0.0f)");
+            // pgabor: periods of the (up to) three spatial coordinates,
+            // zero-padded; a 4th (time) period is ignored as the time
+            // coordinate is (gabornoise.cpp:338-352, opnoise.cpp:558-632)
+            std::string pv = "V3(0.0f)";
+            if (periodic) {
+                std::vector<std::string> pp(pers.begin(), pers.begin() + std::min<size_t>(3, pers.size()));
+                while (pp.size() < 3)
+                    pp.push_back("0.0f");
+                pv = "V3(" + pp[0] + ", " + pp[1] + ", " + pp[2] + ")";
+            }
+            const std::string pargs = std::string(periodic ? "true" : "false") + ", " + pv;
             line("{ const GaborUniform gu_ = " + gus + ";");
             line("  Df gx_ = " + dc[0] + ", gy_ = " + dc[1] + ", gz_ = " + dc[2] + ";");
-            line("  Df r_ = gabor(gu_, Dv(V3(gx_.v, gy_.v, gz_.v), V3(gx_.dx, gy_.dx, gz_.dx), V3(gx_.dy, gy_.dy, gz_.dy)));");
-            line("  " + var(L, r) + " = " + (rk == K_DF ? "r_" : "r_.v") + "; }");
+            const std::string P = "Dv(V3(gx_.v, gy_.v, gz_.v), V3(gx_.dx, gy_.dx, gz_.dx), V3(gx_.dy, gy_.dy, gz_.dy))";
+            if (tr) {
+                line("  Dv r_ = gabor3(gu_, " + P + ", " + pargs + ");");
+                line("  " + var(L, r) + " = " + (rk == K_DV ? "r_" : "r_.v") + "; }");
+            } else {
+                line("  Df r_ = gabor(gu_, " + P + ", " + pargs + ");");
+                line("  " + var(L, r) + " = " + (rk == K_DF ? "r_" : "r_.v") + "; }");
+            }
             return;
         }
         fail("unsupported noise type '" + nm + "'");
@@ -1482,7 +1541,10 @@ struct Emitter {
                  "            }\n"
                  "        }\n";
         }
+        if (g.surface_closure && ss.closure_export)
+            k << "        if (A.clo) export_closure(ci_, A.clo, A.clo_max, A.n, i);\n";
         g.can_sample    = g.surface_closure && ss.closure_sample;
+        g.can_export    = g.surface_closure && ss.closure_export;
         code            = k.str();
         g.uses_colorsys = uses_cs;
         g.uses_mx       = uses_mx;
@@ -1519,6 +1581,7 @@ struct Emitter {
     static std::string prelude(bool mx, bool xp = false)
     {
         return std::string(mx ? "#define OSLMI_MX_CLOSURES 1\n" : "")
+               + std::string(xp ? "#define OSLMI_CLOSURE_EXPORT 1\n" : "")
                + "#include \"oslmi_rt.h\"\n#include \"oslmi_groupargs.h\"\n"
                "using namespace oslmi;\nusing namespace oslmi::rt;\n"
                "__device__ __forceinline__ V3 ld3(const GroupArgs& A, int g, int64_t i) {\n"
@@ -1541,10 +1604,7 @@ bool emit_group(const ShadingSys& ss, Group& g, std::string& src, std::string& e
     return true;
 }
 
-        if (g.surface_closure && ss.closure_export)
-            k << "        if (A.clo) export_closure(ci_, A.clo, A.clo_max, A.n, i);\n";
 // One persistent kernel for a list of groups. Work arrives as tiles
-        g.can_export    = g.surface_closure && ss.closure_export;
 // {item, begin, end} (BatchTile), sorted by group by the host; every
 // workgroup pulls the next tile from a device counter and runs the tile's
 // group body over blockDim.x * ppt consecutive points (ppt strided passes,
@@ -1581,7 +1641,6 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
         c << "        const GroupArgs& A = args[" << gi << "];\n";
         c << "        using Closure = LobeList<" << e.closure_capacity() << ">;\n";
         if (e.uses_cs)
-               + std::string(xp ? "#define OSLMI_CLOSURE_EXPORT 1\n" : "")
             c << "        const ColorSysDev* CS = (const ColorSysDev*)A.colorsys;\n";
         c << "        for (;;) {\n";
         c << "#pragma unroll 1\n";

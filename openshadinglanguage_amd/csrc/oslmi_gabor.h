@@ -105,9 +105,9 @@ OSLMI_DEV Df ddot(V3 a, const Dv& b) { return comp(b, 0) * a.x + comp(b, 1) * a.
 
 struct GaborRng {
     uint32_t s;
-    OSLMI_DEV GaborRng(V3 p)
+    OSLMI_DEV GaborRng(V3 p, uint32_t seed)
     {
-        s = inthash((uint32_t)ifloor(p.x), (uint32_t)ifloor(p.y), (uint32_t)ifloor(p.z), 0u);
+        s = inthash((uint32_t)ifloor(p.x), (uint32_t)ifloor(p.y), (uint32_t)ifloor(p.z), seed);
         if (!s)
             s = 1;
     }
@@ -237,10 +237,17 @@ OSLMI_DEV Df gabor_impulse(const GaborUniform& U, const GaborPoint& gp, V3 omega
 }
 
 // gabor_cell: impulses of one lattice cell, summed in generation order.
-OSLMI_DEV Df gabor_cell(const GaborUniform& U, const GaborPoint& gp, V3 c_i, const Dv& x_c_i)
+// seed selects the generator stream (0 for the scalar noise, 0/1/2 for the
+// components of the vector noise); a periodic noise wraps the cell corner
+// before it seeds the generator (sfm_gabornoise.h:304-310).
+OSLMI_DEV Df gabor_cell(const GaborUniform& U, const GaborPoint& gp, V3 c_i, const Dv& x_c_i,
+                        uint32_t seed, bool periodic, V3 period)
 {
     const float two_pi = 6.28318548202514648438f;
-    GaborRng rng(c_i);
+    if (periodic)
+        c_i = V3(wrap_period(c_i.x, period.x), wrap_period(c_i.y, period.y),
+                 wrap_period(c_i.z, period.z));
+    GaborRng rng(c_i, seed);
     // Poisson(lambda*radius3) by multiplying uniforms (fast_rng::poisson)
     int n = 0;
     {
@@ -289,11 +296,10 @@ OSLMI_DEV Df gabor_cell(const GaborUniform& U, const GaborPoint& gp, V3 c_i, con
     return sum;
 }
 
-// scalar_gabor for a Dual2 point P (dim 1..3 input already packed in P).
-OSLMI_DEV Df gabor(const GaborUniform& U, const Dv& P)
+// gabor_evaluate + gabor_grid: the 27 cells around P, scaled.
+OSLMI_DEV Df gabor_grid(const GaborUniform& U, const GaborPoint& gp, const Dv& P, uint32_t seed,
+                        bool periodic, V3 period)
 {
-    GaborPoint gp;
-    gabor_setup(U, P, gp);
     Dv xg = P * U.radius_inv;
     V3 fl(floorf(xg.v.x), floorf(xg.v.y), floorf(xg.v.z));
     Dv xc(xg.v - fl, xg.dx, xg.dy);
@@ -302,9 +308,42 @@ OSLMI_DEV Df gabor(const GaborUniform& U, const Dv& P)
         for (int j = -1; j <= 1; ++j)
             for (int i = -1; i <= 1; ++i) {
                 V3 c((float)i, (float)j, (float)k);
-                sum = sum + gabor_cell(U, gp, fl + c, Dv(xc.v - c, xc.dx, xc.dy));
+                sum = sum + gabor_cell(U, gp, fl + c, Dv(xc.v - c, xc.dx, xc.dy), seed, periodic,
+                                       period);
             }
     return (sum * U.sqrt_lambda_inv) * U.scale;
+}
+
+// scalar_gabor / scalar_pgabor for a Dual2 point P (dim 1..3 input already
+// packed in P; pgabor pads the periods of 1 and 2 dimensions with zeros).
+OSLMI_DEV Df gabor(const GaborUniform& U, const Dv& P, bool periodic = false,
+                   V3 period = V3(0.0f))
+{
+    GaborPoint gp;
+    gabor_setup(U, P, gp);
+    return gabor_grid(U, gp, P, 0u, periodic, period);
+}
+
+// scalar_gabor3 / scalar_pgabor3 (sfm_gabornoise.h:517-640): one filter
+// set-up, the three components from generator seeds 0, 1 and 2. The seed
+// loop is kept rolled: the cell walk is large and runs once per component.
+OSLMI_DEV Dv gabor3(const GaborUniform& U, const Dv& P, bool periodic = false,
+                    V3 period = V3(0.0f))
+{
+    GaborPoint gp;
+    gabor_setup(U, P, gp);
+    Df r0, r1, r2;
+#pragma clang loop unroll(disable)
+    for (uint32_t seed = 0; seed < 3; ++seed) {
+        const Df r = gabor_grid(U, gp, P, seed, periodic, period);
+        if (seed == 0)
+            r0 = r;
+        else if (seed == 1)
+            r1 = r;
+        else
+            r2 = r;
+    }
+    return make_dv(r0, r1, r2);
 }
 
 }  // namespace oslmi

@@ -144,6 +144,57 @@ OSLMI_DEV int hash_op4(float x, float y, float z, float w)
 }
 OSLMI_DEV int hash_opi(int x) { return (int)inthash((uint32_t)x); }
 
+// ------------------------------------------------------- periodic lattice
+// imod (oslnoise.h:847-859): the remainder of a / b in [0, b) for b > 0.
+OSLMI_DEV int imod(int a, int b)
+{
+    int r = a % b;
+    if (r < 0)
+        r += b;
+    return r;
+}
+
+// The lattice period of a float period argument (HashScalarPeriodic,
+// oslnoise.h:1162-1180): ifloor(p), at least 1. A floor that no int holds
+// (NaN, 2^31 and beyond) gives 1, as x86's truncating conversion (INT_MIN)
+// followed by the clamp does on the CPU.
+OSLMI_DEV int period_of(float p)
+{
+    const float f = floorf(p);
+    return (f >= 1.0f && f < 2147483648.0f) ? (int)f : 1;
+}
+
+// PeriodicAdaptionOf::wrap (oslnoise.h:532-539) and gabornoise.h:185-192:
+// s reduced into [0, floor(period)), the period at least 1. Periodic cell
+// and hash noise hash the wrapped coordinates.
+OSLMI_DEV float wrap_period(float s, float period)
+{
+    period = floorf(period);
+    if (period < 1.0f)
+        period = 1.0f;
+    return s - period * floorf(s / period);
+}
+
+// Lattice-coordinate maps applied before hashing: NoWrap for the plain
+// noises, Wrap (imod by the per-axis lattice period) for the periodic ones
+// (HashScalarPeriodic / HashVectorPeriodic, oslnoise.h:1157-1290).
+struct NoWrap {
+    OSLMI_DEV uint32_t operator()(int x, int) const { return (uint32_t)x; }
+};
+struct Wrap {
+    int p[4];
+    OSLMI_DEV uint32_t operator()(int x, int axis) const { return (uint32_t)imod(x, p[axis]); }
+};
+OSLMI_DEV Wrap make_wrap(float a, float b = 1.0f, float c = 1.0f, float d = 1.0f)
+{
+    Wrap w;
+    w.p[0] = period_of(a);
+    w.p[1] = period_of(b);
+    w.p[2] = period_of(c);
+    w.p[3] = period_of(d);
+    return w;
+}
+
 // ----------------------------------------------------------------- perlin
 template<class T> struct NOps;
 template<> struct NOps<float> {
@@ -227,24 +278,25 @@ template<class T> OSLMI_DEV T grad4(int hash, const T& x, const T& y, const T& z
 // are the sliceup() fields of HashVector. grad() reads at most 5 low bits.
 OSLMI_DEV int hslice(uint32_t h, int shift) { return (int)(h >> shift); }
 
-template<class T> OSLMI_DEV T perlin1(const T& x, int shift)
+template<class T, class W = NoWrap> OSLMI_DEV T perlin1(const T& x, int shift, const W& wr = W())
 {
     int X;
     T fx = floorfrac(x, &X);
     T u  = fade(fx);
-    T r  = lerp(grad1(hslice(inthash((uint32_t)X), shift), fx),
-                grad1(hslice(inthash((uint32_t)(X + 1)), shift), fx - 1.0f), u);
+    T r  = lerp(grad1(hslice(inthash(wr(X, 0)), shift), fx),
+                grad1(hslice(inthash(wr(X + 1, 0)), shift), fx - 1.0f), u);
     return 0.25f * r;
 }
 
-template<class T> OSLMI_DEV T perlin2(const T& x, const T& y, int shift)
+template<class T, class W = NoWrap>
+OSLMI_DEV T perlin2(const T& x, const T& y, int shift, const W& wr = W())
 {
     int X, Y;
     T fx = floorfrac(x, &X);
     T fy = floorfrac(y, &Y);
     T u  = fade(fx);
     T v  = fade(fy);
-    uint32_t X0 = (uint32_t)X, X1 = (uint32_t)(X + 1), Y0 = (uint32_t)Y, Y1 = (uint32_t)(Y + 1);
+    uint32_t X0 = wr(X, 0), X1 = wr(X + 1, 0), Y0 = wr(Y, 1), Y1 = wr(Y + 1, 1);
     T r = bilerp(grad2(hslice(inthash(X0, Y0), shift), fx, fy),
                  grad2(hslice(inthash(X1, Y0), shift), fx - 1.0f, fy),
                  grad2(hslice(inthash(X0, Y1), shift), fx, fy - 1.0f),
@@ -258,11 +310,11 @@ template<class T> OSLMI_DEV T perlin2(const T& x, const T& y, int shift)
 struct Corners3 {
     uint32_t h[8];
 };
-OSLMI_DEV Corners3 corners3(int X, int Y, int Z)
+template<class W = NoWrap> OSLMI_DEV Corners3 corners3(int X, int Y, int Z, const W& wr = W())
 {
     Corners3 c;
-    uint32_t X0 = (uint32_t)X, X1 = (uint32_t)(X + 1), Y0 = (uint32_t)Y, Y1 = (uint32_t)(Y + 1),
-             Z0 = (uint32_t)Z, Z1 = (uint32_t)(Z + 1);
+    uint32_t X0 = wr(X, 0), X1 = wr(X + 1, 0), Y0 = wr(Y, 1), Y1 = wr(Y + 1, 1), Z0 = wr(Z, 2),
+             Z1 = wr(Z + 1, 2);
     c.h[0] = inthash(X0, Y0, Z0);
     c.h[1] = inthash(X1, Y0, Z0);
     c.h[2] = inthash(X0, Y1, Z0);
@@ -290,34 +342,37 @@ OSLMI_DEV T perlin3_eval(const Corners3& c, int shift, const T& fx, const T& fy,
     return 0.9820f * r;
 }
 
-template<class T> OSLMI_DEV T perlin3(const T& x, const T& y, const T& z)
+template<class T, class W = NoWrap>
+OSLMI_DEV T perlin3(const T& x, const T& y, const T& z, const W& wr = W())
 {
     int X, Y, Z;
     T fx = floorfrac(x, &X), fy = floorfrac(y, &Y), fz = floorfrac(z, &Z);
-    Corners3 c = corners3(X, Y, Z);
+    Corners3 c = corners3(X, Y, Z, wr);
     return perlin3_eval(c, 0, fx, fy, fz, fade(fx), fade(fy), fade(fz));
 }
 
 // Vector perlin: the three components as float/Df results.
-template<class T> OSLMI_DEV void vperlin3(const T& x, const T& y, const T& z, T r[3])
+template<class T, class W = NoWrap>
+OSLMI_DEV void vperlin3(const T& x, const T& y, const T& z, T r[3], const W& wr = W())
 {
     int X, Y, Z;
     T fx = floorfrac(x, &X), fy = floorfrac(y, &Y), fz = floorfrac(z, &Z);
     T u = fade(fx), v = fade(fy), w = fade(fz);
-    Corners3 c = corners3(X, Y, Z);
+    Corners3 c = corners3(X, Y, Z, wr);
     r[0]       = perlin3_eval(c, 0, fx, fy, fz, u, v, w);
     r[1]       = perlin3_eval(c, 8, fx, fy, fz, u, v, w);
     r[2]       = perlin3_eval(c, 16, fx, fy, fz, u, v, w);
 }
 
-template<class T> OSLMI_DEV T perlin4(const T& x, const T& y, const T& z, const T& w, int shift)
+template<class T, class WR = NoWrap>
+OSLMI_DEV T perlin4(const T& x, const T& y, const T& z, const T& w, int shift, const WR& wr = WR())
 {
     int X, Y, Z, W;
     T fx = floorfrac(x, &X), fy = floorfrac(y, &Y), fz = floorfrac(z, &Z), fw = floorfrac(w, &W);
     T u = fade(fx), v = fade(fy), t = fade(fz), s = fade(fw);
     T fx1 = fx - 1.0f, fy1 = fy - 1.0f, fz1 = fz - 1.0f, fw1 = fw - 1.0f;
-    uint32_t X0 = (uint32_t)X, X1 = (uint32_t)(X + 1), Y0 = (uint32_t)Y, Y1 = (uint32_t)(Y + 1),
-             Z0 = (uint32_t)Z, Z1 = (uint32_t)(Z + 1), W0 = (uint32_t)W, W1 = (uint32_t)(W + 1);
+    uint32_t X0 = wr(X, 0), X1 = wr(X + 1, 0), Y0 = wr(Y, 1), Y1 = wr(Y + 1, 1), Z0 = wr(Z, 2),
+             Z1 = wr(Z + 1, 2), W0 = wr(W, 3), W1 = wr(W + 1, 3);
 #define OSLMI_G4(a, b, c, d, px, py, pz, pw) grad4(hslice(inthash(a, b, c, d), shift), px, py, pz, pw)
     T lo = trilerp(OSLMI_G4(X0, Y0, Z0, W0, fx, fy, fz, fw), OSLMI_G4(X1, Y0, Z0, W0, fx1, fy, fz, fw),
                    OSLMI_G4(X0, Y1, Z0, W0, fx, fy1, fz, fw), OSLMI_G4(X1, Y1, Z0, W0, fx1, fy1, fz, fw),

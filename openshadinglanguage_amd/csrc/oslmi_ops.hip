@@ -16,8 +16,40 @@ namespace oslmi {
 
 enum NoiseKind { kPerlin = 0, kUPerlin, kCell, kHash, kSimplex, kUSimplex };
 
-template<int KIND, int OUTDIM, int INDIM>
+// Perlin of INDIM coordinates, OUTDIM components (8-bit slices of one hash
+// per corner for the vector form), over the lattice map W.
+template<class T, int OUTDIM, int INDIM, class W>
+__device__ __forceinline__ void perlin_n(const T* p, T* r, const W& w)
+{
+    if (INDIM == 3 && OUTDIM == 3) {
+        vperlin3<T, W>(p[0], p[1], p[2], r, w);
+    } else {
+#pragma unroll
+        for (int c = 0; c < OUTDIM; ++c) {
+            int sh = 8 * c;
+            r[c]   = INDIM == 1   ? perlin1<T, W>(p[0], sh, w)
+                     : INDIM == 2 ? perlin2<T, W>(p[0], p[1], sh, w)
+                     : INDIM == 3 ? perlin3<T, W>(p[0], p[1], p[2], w)
+                                  : perlin4<T, W>(p[0], p[1], p[2], p[3], sh, w);
+        }
+    }
+}
+
+// the lattice periods of point i (PER) or the identity map
+template<int INDIM>
+__device__ __forceinline__ Wrap point_wrap(const float* __restrict__ per, int64_t n, int64_t i)
+{
+    float q[4] = { 1.f, 1.f, 1.f, 1.f };
+#pragma unroll
+    for (int c = 0; c < INDIM; ++c)
+        q[c] = per[c * n + i];
+    return make_wrap(q[0], q[1], q[2], q[3]);
+}
+
+// PER: periodic form, per holds INDIM period planes (no derivatives).
+template<int KIND, int OUTDIM, int INDIM, bool PER>
 __global__ __launch_bounds__(256) void k_noise(const float* __restrict__ in,
+                                               const float* __restrict__ per,
                                                float* __restrict__ out, int64_t n)
 {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -29,6 +61,10 @@ __global__ __launch_bounds__(256) void k_noise(const float* __restrict__ in,
         float r[3] = { 0.f, 0.f, 0.f };
         if (KIND == kCell || KIND == kHash) {
             constexpr bool CELL = (KIND == kCell);
+            if (PER)
+#pragma unroll
+                for (int c = 0; c < INDIM; ++c)
+                    p[c] = wrap_period(p[c], per[c * n + i]);
             if (OUTDIM == 1) {
                 r[0] = INDIM == 1   ? intnoise1<CELL>(p[0])
                        : INDIM == 2 ? intnoise2<CELL>(p[0], p[1])
@@ -44,18 +80,10 @@ __global__ __launch_bounds__(256) void k_noise(const float* __restrict__ in,
                 r[2] = v.z;
             }
         } else if (KIND == kPerlin || KIND == kUPerlin) {
-            if (INDIM == 3 && OUTDIM == 3) {
-                vperlin3<float>(p[0], p[1], p[2], r);
-            } else {
-#pragma unroll
-                for (int c = 0; c < OUTDIM; ++c) {
-                    int sh = 8 * c;
-                    r[c]   = INDIM == 1   ? perlin1<float>(p[0], sh)
-                             : INDIM == 2 ? perlin2<float>(p[0], p[1], sh)
-                             : INDIM == 3 ? perlin3<float>(p[0], p[1], p[2])
-                                          : perlin4<float>(p[0], p[1], p[2], p[3], sh);
-                }
-            }
+            if (PER)
+                perlin_n<float, OUTDIM, INDIM>(p, r, point_wrap<INDIM>(per, n, i));
+            else
+                perlin_n<float, OUTDIM, INDIM>(p, r, NoWrap());
             if (KIND == kUPerlin)
 #pragma unroll
                 for (int c = 0; c < OUTDIM; ++c)
@@ -78,8 +106,9 @@ __global__ __launch_bounds__(256) void k_noise(const float* __restrict__ in,
 }
 
 // Dual2 variants (value + dx + dy planes in and out).
-template<int KIND, int OUTDIM, int INDIM>
+template<int KIND, int OUTDIM, int INDIM, bool PER>
 __global__ __launch_bounds__(256) void k_noise_d(const float* __restrict__ in,
+                                                 const float* __restrict__ per,
                                                  float* __restrict__ out, int64_t n)
 {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -90,18 +119,10 @@ __global__ __launch_bounds__(256) void k_noise_d(const float* __restrict__ in,
             p[c] = Df(in[c * n + i], in[(INDIM + c) * n + i], in[(2 * INDIM + c) * n + i]);
         Df r[3];
         if (KIND == kPerlin || KIND == kUPerlin) {
-            if (INDIM == 3 && OUTDIM == 3) {
-                vperlin3<Df>(p[0], p[1], p[2], r);
-            } else {
-#pragma unroll
-                for (int c = 0; c < OUTDIM; ++c) {
-                    int sh = 8 * c;
-                    r[c]   = INDIM == 1   ? perlin1<Df>(p[0], sh)
-                             : INDIM == 2 ? perlin2<Df>(p[0], p[1], sh)
-                             : INDIM == 3 ? perlin3<Df>(p[0], p[1], p[2])
-                                          : perlin4<Df>(p[0], p[1], p[2], p[3], sh);
-                }
-            }
+            if (PER)
+                perlin_n<Df, OUTDIM, INDIM>(p, r, point_wrap<INDIM>(per, n, i));
+            else
+                perlin_n<Df, OUTDIM, INDIM>(p, r, NoWrap());
             if (KIND == kUPerlin)
 #pragma unroll
                 for (int c = 0; c < OUTDIM; ++c)
@@ -159,33 +180,33 @@ __global__ __launch_bounds__(256) void k_hash(const float* __restrict__ in, int3
     }
 }
 
-typedef void (*NoiseKernel)(const float*, float*, int64_t);
+typedef void (*NoiseKernel)(const float*, const float*, float*, int64_t);
 
-template<int KIND, int OUTDIM> static NoiseKernel pick_plain(int indim)
+template<int KIND, int OUTDIM, bool PER> static NoiseKernel pick_plain(int indim)
 {
     switch (indim) {
-    case 1: return k_noise<KIND, OUTDIM, 1>;
-    case 2: return k_noise<KIND, OUTDIM, 2>;
-    case 3: return k_noise<KIND, OUTDIM, 3>;
-    case 4: return k_noise<KIND, OUTDIM, 4>;
+    case 1: return k_noise<KIND, OUTDIM, 1, PER>;
+    case 2: return k_noise<KIND, OUTDIM, 2, PER>;
+    case 3: return k_noise<KIND, OUTDIM, 3, PER>;
+    case 4: return k_noise<KIND, OUTDIM, 4, PER>;
     }
     return nullptr;
 }
-template<int KIND, int OUTDIM> static NoiseKernel pick_deriv(int indim)
+template<int KIND, int OUTDIM, bool PER> static NoiseKernel pick_deriv(int indim)
 {
     switch (indim) {
-    case 1: return k_noise_d<KIND, OUTDIM, 1>;
-    case 2: return k_noise_d<KIND, OUTDIM, 2>;
-    case 3: return k_noise_d<KIND, OUTDIM, 3>;
-    case 4: return k_noise_d<KIND, OUTDIM, 4>;
+    case 1: return k_noise_d<KIND, OUTDIM, 1, PER>;
+    case 2: return k_noise_d<KIND, OUTDIM, 2, PER>;
+    case 3: return k_noise_d<KIND, OUTDIM, 3, PER>;
+    case 4: return k_noise_d<KIND, OUTDIM, 4, PER>;
     }
     return nullptr;
 }
-template<int KIND> static NoiseKernel pick(int outdim, int indim, bool derivs)
+template<int KIND, bool PER = false> static NoiseKernel pick(int outdim, int indim, bool derivs)
 {
     if (derivs)
-        return outdim == 1 ? pick_deriv<KIND, 1>(indim) : pick_deriv<KIND, 3>(indim);
-    return outdim == 1 ? pick_plain<KIND, 1>(indim) : pick_plain<KIND, 3>(indim);
+        return outdim == 1 ? pick_deriv<KIND, 1, PER>(indim) : pick_deriv<KIND, 3, PER>(indim);
+    return outdim == 1 ? pick_plain<KIND, 1, PER>(indim) : pick_plain<KIND, 3, PER>(indim);
 }
 
 }  // namespace oslmi
@@ -232,17 +253,54 @@ extern "C" int oslmi_noise(const char* name, int outdim, int indim, int derivs, 
     if (!k)
         return set_error(OSLMI_ERR_UNSUPPORTED, "oslmi_noise: unsupported dimension for " + s);
     hipStream_t st = (hipStream_t)stream;
-    const float* src = d_in;
-    if (derivs && !dv) {
-        // value planes of a Dual2 input are its first indim planes
-        src = d_in;
-    }
+    // the value planes of a Dual2 input are its first indim planes
     dim3 grid(grid_for(n)), block(256);
-    hipLaunchKernelGGL(k, grid, block, 0, st, src, d_out, n);
+    hipLaunchKernelGGL(k, grid, block, 0, st, d_in, nullptr, d_out, n);
     if (derivs && !dv)
         hipLaunchKernelGGL(k_zero, dim3(grid_for(2 * outdim * n)), block, 0, st,
                            d_out + outdim * n, 2 * outdim * n);
     return check_launch("oslmi_noise");
+}
+
+extern "C" int oslmi_pnoise(const char* name, int outdim, int indim, int derivs, const float* d_in,
+                            const float* d_period, float* d_out, int64_t n, void* stream)
+{
+    if (!name || n < 0 || (outdim != 1 && outdim != 3) || indim < 1 || indim > 4)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_pnoise: bad arguments");
+    if (n == 0)
+        return OSLMI_OK;
+    if (!d_in || !d_period || !d_out)
+        return set_error(OSLMI_ERR_INVALID, "oslmi_pnoise: null buffer");
+    std::string s(name);
+    int kind;
+    if (s == "perlin" || s == "snoise" || s == "psnoise")
+        kind = kPerlin;
+    else if (s == "uperlin" || s == "noise" || s == "pnoise")
+        kind = kUPerlin;
+    else if (s == "cell" || s == "cellnoise")
+        kind = kCell;
+    else if (s == "hash" || s == "hashnoise")
+        kind = kHash;
+    else
+        return set_error(OSLMI_ERR_UNSUPPORTED, "oslmi_pnoise: periodic noise type '" + s
+                                                    + "' is unknown");
+    bool dv = derivs && kind != kCell && kind != kHash;
+    NoiseKernel k = nullptr;
+    switch (kind) {
+    case kPerlin: k = pick<kPerlin, true>(outdim, indim, dv); break;
+    case kUPerlin: k = pick<kUPerlin, true>(outdim, indim, dv); break;
+    case kCell: k = pick<kCell, true>(outdim, indim, false); break;
+    case kHash: k = pick<kHash, true>(outdim, indim, false); break;
+    }
+    if (!k)
+        return set_error(OSLMI_ERR_UNSUPPORTED, "oslmi_pnoise: unsupported dimension for " + s);
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid(grid_for(n)), block(256);
+    hipLaunchKernelGGL(k, grid, block, 0, st, d_in, d_period, d_out, n);
+    if (derivs && !dv)
+        hipLaunchKernelGGL(k_zero, dim3(grid_for(2 * outdim * n)), block, 0, st,
+                           d_out + outdim * n, 2 * outdim * n);
+    return check_launch("oslmi_pnoise");
 }
 
 extern "C" int oslmi_hash(int indim, const float* d_in, int32_t* d_out, int64_t n, void* stream)

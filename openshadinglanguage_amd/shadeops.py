@@ -38,6 +38,23 @@ def noise(name, outdim, inp, derivs=False):
     return out
 
 
+def pnoise(name, outdim, inp, period, derivs=False):
+    """pnoise(name, P, period) over n points. inp as for noise(); period:
+    [indim, n] float32 (one period per input dimension and point)."""
+    _planes(inp)
+    _planes(period)
+    planes, n = inp.shape
+    indim = planes // 3 if derivs else planes
+    if derivs and planes % 3:
+        raise ValueError("Dual2 input needs 3*indim planes")
+    if tuple(period.shape) != (indim, n):
+        raise ValueError("period needs [indim, n] planes")
+    out = torch.empty(((3 if derivs else 1) * outdim, n), device=inp.device, dtype=torch.float32)
+    check(lib().oslmi_pnoise(name.encode(), outdim, indim, int(derivs), _ptr(inp), _ptr(period),
+                             _ptr(out), n, _stream()))
+    return out
+
+
 def hash_op(inp):
     _planes(inp)
     out = torch.empty(inp.shape[1], device=inp.device, dtype=torch.int32)

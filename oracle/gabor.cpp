@@ -280,10 +280,15 @@ void setup_filter(const D2<V3>& P, Params& gp)
     gp.do_filter = do_filter;
 }
 
+// gabor_cell (sfm_gabornoise.h:304-380): with a period the cell's corner
+// is wrapped (gabornoise.h:185-199) before it seeds the generator.
 D2<float> cell(const GaborUniform& gup, const GaborOpts& opt, const Params& gp, V3 c_i,
-               const D2<V3>& x_c_i)
+               const D2<V3>& x_c_i, int seed, const V3* period)
 {
-    FastRng rng(c_i, 0);
+    if (period)
+        c_i = V3(wrap_period(c_i.x, period->x), wrap_period(c_i.y, period->y),
+                 wrap_period(c_i.z, period->z));
+    FastRng rng(c_i, seed);
     int n_imp     = rng.poisson(gup.poisson_g);
     D2<float> sum = D2<float>(0.0f);
     for (int i = 0; i < n_imp; ++i) {
@@ -372,8 +377,27 @@ GaborUniform gabor_uniform(const GaborOpts& opt)
 
 D2<float> gabor(int dim, const D2<float>* p, const GaborOpts& opt)
 {
+    return gabor_ex(dim, p, opt, 0, nullptr);
+}
+
+D2<V3> gabor3(int dim, const D2<float>* p, const GaborOpts& opt, const float* period)
+{
+    D2<float> r[3];
+    for (int seed = 0; seed < 3; ++seed)
+        r[seed] = gabor_ex(dim, p, opt, seed, period);
+    return D2<V3>(V3(r[0].v, r[1].v, r[2].v), V3(r[0].dx, r[1].dx, r[2].dx),
+                  V3(r[0].dy, r[1].dy, r[2].dy));
+}
+
+D2<float> gabor_ex(int dim, const D2<float>* p, const GaborOpts& opt, int seed,
+                   const float* period)
+{
     GaborUniform gup = gabor_uniform(opt);
     const float zero = 0.0f;
+    // pgabor of 1 and 2 dimensions pads the period with zeros
+    // (gabornoise.cpp:338-352)
+    V3 per(period ? period[0] : zero, period && dim > 1 ? period[1] : zero,
+           period && dim > 2 ? period[2] : zero);
     D2<V3> P(V3(p[0].v, dim > 1 ? p[1].v : zero, dim > 2 ? p[2].v : zero),
              V3(p[0].dx, dim > 1 ? p[1].dx : zero, dim > 2 ? p[2].dx : zero),
              V3(p[0].dy, dim > 1 ? p[1].dy : zero, dim > 2 ? p[2].dy : zero));
@@ -398,7 +422,7 @@ D2<float> gabor(int dim, const D2<float>* p, const GaborOpts& opt)
                 V3 c((float)i, (float)j, (float)k);
                 V3 c_i = fl + c;
                 D2<V3> x_c_i(xc.v - c, xc.dx, xc.dy);
-                sum = sum + cell(gup, opt, gp, c_i, x_c_i);
+                sum = sum + cell(gup, opt, gp, c_i, x_c_i, seed, period ? &per : nullptr);
             }
     D2<float> result   = sum * gup.sqrt_lambda_inv;
     float a3           = gup.a * gup.a * gup.a;

@@ -24,5 +24,13 @@ GaborUniform gabor_uniform(const GaborOpts& opt);
 // scalar_gabor (sfm_gabornoise.h:497-515) for a Dual2 point; dim 1..3
 // builds P as (x,0,0), (x,y,0) or (x,y,z) (make_Vec3, wide_gabornoise.h).
 D2<float> gabor(int dim, const D2<float>* p, const GaborOpts& opt);
+// The same with the generator seed (0 for the scalar noise, 0/1/2 for the
+// components of gabor3) and optional periods (pgabor; one float per input
+// dimension, null for none).
+D2<float> gabor_ex(int dim, const D2<float>* p, const GaborOpts& opt, int seed,
+                   const float* period);
+// scalar_gabor3 / scalar_pgabor3 (sfm_gabornoise.h:517-640): the three
+// components are the scalar evaluation with seeds 0, 1 and 2.
+D2<V3> gabor3(int dim, const D2<float>* p, const GaborOpts& opt, const float* period);
 
 }  // namespace oref

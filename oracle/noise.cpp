@@ -199,20 +199,29 @@ static inline T grad(int hash, const T& x, const T& y, const T& z, const T& w)
 // HashScalar value, comps 1/2 take (h>>8)&0xFF and (h>>16)&0xFF as
 // HashVector/sliceup does. grad() only looks at the low 5 bits, so the
 // component-0 mask (h & 0xFF) does not change anything.
+//
+// With per-axis periods (HashScalarPeriodic / HashVectorPeriodic,
+// oslnoise.h:1157-1290) each lattice coordinate is first reduced with imod
+// (oslnoise.h:847-859) to [0, period).
 struct LatHash {
     int shift;
-    int operator()(int x) const { return sl(inthash((uint32_t)x)); }
+    const int* per = nullptr;  // periods (>= 1) per axis, or null
+    uint32_t w(int x, int axis) const
+    {
+        return per ? (uint32_t)imod(x, per[axis]) : (uint32_t)x;
+    }
+    int operator()(int x) const { return sl(inthash(w(x, 0))); }
     int operator()(int x, int y) const
     {
-        return sl(inthash((uint32_t)x, (uint32_t)y));
+        return sl(inthash(w(x, 0), w(y, 1)));
     }
     int operator()(int x, int y, int z) const
     {
-        return sl(inthash((uint32_t)x, (uint32_t)y, (uint32_t)z));
+        return sl(inthash(w(x, 0), w(y, 1), w(z, 2)));
     }
-    int operator()(int x, int y, int z, int w) const
+    int operator()(int x, int y, int z, int ww) const
     {
-        return sl(inthash((uint32_t)x, (uint32_t)y, (uint32_t)z, (uint32_t)w));
+        return sl(inthash(w(x, 0), w(y, 1), w(z, 2), w(ww, 3)));
     }
     int sl(uint32_t h) const
     {
@@ -329,9 +338,10 @@ static T perlin4(const LatHash& hash, const T& x, const T& y, const T& z,
     return 0.8344f * lerp(a, b, s);
 }
 
-template<class T, bool SIMD = false> static T perlin_n(int dim, const T* p, int shift)
+template<class T, bool SIMD = false>
+static T perlin_n(int dim, const T* p, int shift, const int* per = nullptr)
 {
-    LatHash h { shift };
+    LatHash h { shift, per };
     switch (dim) {
     case 1: return perlin1(h, p[0]);
     case 2: return perlin2(h, p[0], p[1]);
@@ -386,6 +396,99 @@ D2<V3> unoise_dv(int dim, const D2<float>* p)
                      0.5f * (r.v.z + 1.0f)),
                   V3(0.5f * r.dx.x, 0.5f * r.dx.y, 0.5f * r.dx.z),
                   V3(0.5f * r.dy.x, 0.5f * r.dy.y, 0.5f * r.dy.z));
+}
+
+// ---------------------------------------------------------------------------
+// Periodic variants. PeriodicSNoise / PeriodicNoise (oslnoise.h:2531-2760)
+// are the perlin above over the periodic lattice hash; the periods are
+// plain floats (no derivatives), turned into lattice periods by
+// period_of. PeriodicCellNoise / PeriodicHashNoise (PeriodicAdaptionOf,
+// oslnoise.h:486-548) wrap the float coordinates with wrap_period and then
+// hash as the plain versions do.
+static void lattice_periods(int dim, const float* period, int per[4])
+{
+    for (int i = 0; i < dim; ++i)
+        per[i] = period_of(period[i]);
+}
+
+float psnoise_f(int dim, const float* p, const float* period)
+{
+    int per[4];
+    lattice_periods(dim, period, per);
+    return perlin_n<float>(dim, p, 0, per);
+}
+V3 psnoise_v(int dim, const float* p, const float* period)
+{
+    int per[4];
+    lattice_periods(dim, period, per);
+    return V3(perlin_n<float>(dim, p, 0, per), perlin_n<float>(dim, p, 8, per),
+              perlin_n<float>(dim, p, 16, per));
+}
+D2<float> psnoise_df(int dim, const D2<float>* p, const float* period)
+{
+    int per[4];
+    lattice_periods(dim, period, per);
+    return perlin_n<D2<float>>(dim, p, 0, per);
+}
+D2<V3> psnoise_dv(int dim, const D2<float>* p, const float* period)
+{
+    int per[4];
+    lattice_periods(dim, period, per);
+    D2<float> a = perlin_n<D2<float>>(dim, p, 0, per);
+    D2<float> b = perlin_n<D2<float>>(dim, p, 8, per);
+    D2<float> c = perlin_n<D2<float>>(dim, p, 16, per);
+    return D2<V3>(V3(a.v, b.v, c.v), V3(a.dx, b.dx, c.dx), V3(a.dy, b.dy, c.dy));
+}
+float pnoise_f(int dim, const float* p, const float* period)
+{
+    return 0.5f * (psnoise_f(dim, p, period) + 1.0f);
+}
+V3 pnoise_v(int dim, const float* p, const float* period)
+{
+    V3 r = psnoise_v(dim, p, period);
+    return V3(0.5f * (r.x + 1.0f), 0.5f * (r.y + 1.0f), 0.5f * (r.z + 1.0f));
+}
+D2<float> pnoise_df(int dim, const D2<float>* p, const float* period)
+{
+    return 0.5f * (psnoise_df(dim, p, period) + 1.0f);
+}
+D2<V3> pnoise_dv(int dim, const D2<float>* p, const float* period)
+{
+    D2<V3> r = psnoise_dv(dim, p, period);
+    return D2<V3>(V3(0.5f * (r.v.x + 1.0f), 0.5f * (r.v.y + 1.0f),
+                     0.5f * (r.v.z + 1.0f)),
+                  V3(0.5f * r.dx.x, 0.5f * r.dx.y, 0.5f * r.dx.z),
+                  V3(0.5f * r.dy.x, 0.5f * r.dy.y, 0.5f * r.dy.z));
+}
+
+static void wrap_coords(int dim, const float* p, const float* period, float* q)
+{
+    for (int i = 0; i < dim; ++i)
+        q[i] = wrap_period(p[i], period[i]);
+}
+float pcellnoise_f(int dim, const float* p, const float* period)
+{
+    float q[4];
+    wrap_coords(dim, p, period, q);
+    return cellnoise_f(dim, q);
+}
+V3 pcellnoise_v(int dim, const float* p, const float* period)
+{
+    float q[4];
+    wrap_coords(dim, p, period, q);
+    return cellnoise_v(dim, q);
+}
+float phashnoise_f(int dim, const float* p, const float* period)
+{
+    float q[4];
+    wrap_coords(dim, p, period, q);
+    return hashnoise_f(dim, q);
+}
+V3 phashnoise_v(int dim, const float* p, const float* period)
+{
+    float q[4];
+    wrap_coords(dim, p, period, q);
+    return hashnoise_v(dim, q);
 }
 
 // ---------------------------------------------------------------------------

@@ -80,6 +80,35 @@ inline float bits_to_01(uint32_t bits)
 // OIIO::ifloor
 inline int ifloor(float x) { return (int)floorf(x); }
 
+// imod (oslnoise.h:847-859): the remainder of a / b in [0, b) for b > 0.
+inline int imod(int a, int b)
+{
+    int r = a % b;
+    if (r < 0)
+        r += b;
+    return r;
+}
+
+// The lattice period of a float period argument (HashScalarPeriodic,
+// oslnoise.h:1162-1180): ifloor(p), at least 1. A floor that no int holds
+// (NaN, beyond 2^31) gives 1, as the x86 truncating conversion (INT_MIN)
+// followed by the clamp does.
+inline int period_of(float p)
+{
+    float f = floorf(p);
+    return (f >= 1.0f && f < 2147483648.0f) ? (int)f : 1;
+}
+
+// PeriodicAdaptionOf::wrap (oslnoise.h:532-539; gabornoise.h:185-192):
+// s reduced into [0, floor(period)) with the period at least 1.
+inline float wrap_period(float s, float period)
+{
+    period = floorf(period);
+    if (period < 1.0f)
+        period = 1.0f;
+    return s - period * floorf(s / period);
+}
+
 // --- cell / hash noise (oslnoise.h:337-480) --------------------------------
 // dim in 1..4 (the 4th coordinate is the 't' argument). vec selects the
 // Vec3-valued variant, which hashes an extra trailing word 0,1,2.
@@ -110,6 +139,21 @@ float unoise_f(int dim, const float* p);
 V3 unoise_v(int dim, const float* p);
 D2<float> unoise_df(int dim, const D2<float>* p);
 D2<V3> unoise_dv(int dim, const D2<float>* p);
+
+// Periodic perlin ("psnoise" signed, "pnoise" unsigned) and periodic
+// cell/hash noise; period holds one float per input dimension.
+float psnoise_f(int dim, const float* p, const float* period);
+V3 psnoise_v(int dim, const float* p, const float* period);
+D2<float> psnoise_df(int dim, const D2<float>* p, const float* period);
+D2<V3> psnoise_dv(int dim, const D2<float>* p, const float* period);
+float pnoise_f(int dim, const float* p, const float* period);
+V3 pnoise_v(int dim, const float* p, const float* period);
+D2<float> pnoise_df(int dim, const D2<float>* p, const float* period);
+D2<V3> pnoise_dv(int dim, const D2<float>* p, const float* period);
+float pcellnoise_f(int dim, const float* p, const float* period);
+V3 pcellnoise_v(int dim, const float* p, const float* period);
+float phashnoise_f(int dim, const float* p, const float* period);
+V3 phashnoise_v(int dim, const float* p, const float* period);
 
 // --- simplex (sfm_simplex.h, SimplexNoiseScalar oslnoise.h:2878-2990) -----
 float simplex_f(int dim, const float* p, int seed, float* dnoise);

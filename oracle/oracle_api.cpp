@@ -194,6 +194,72 @@ int oref_noise(const char* name, int outdim, int indim, int derivs,
     return 0;
 }
 
+// Periodic noise over SoA planes: period holds indim planes of n floats
+// (no derivatives). Names map as the periodic branch of llvm_gen_noise
+// (batched_llvm_gen.cpp:6096-6134): perlin/snoise/psnoise -> psnoise,
+// uperlin/noise/pnoise -> pnoise, cell -> pcellnoise, hash -> phashnoise;
+// simplex has no periodic form (-1).
+int oref_pnoise(const char* name, int outdim, int indim, int derivs, const float* in,
+                const float* period, float* out, int64_t n)
+{
+    std::string s(name);
+    NoiseKind kind = (s == "psnoise") ? kPerlin : (s == "pnoise") ? kUPerlin : noise_kind(name);
+    if (kind == kBad || kind == kSimplex || kind == kUSimplex || indim < 1 || indim > 4
+        || (outdim != 1 && outdim != 3))
+        return -1;
+    bool dv = derivs && kind != kCell && kind != kHash;
+    for (int64_t i = 0; i < n; ++i) {
+        float per[4];
+        for (int c = 0; c < indim; ++c)
+            per[c] = period[c * n + i];
+        if (!dv) {
+            float p[4];
+            for (int c = 0; c < indim; ++c)
+                p[c] = in[c * n + i];
+            V3 v(0.0f);
+            if (outdim == 1) {
+                switch (kind) {
+                case kPerlin: v.x = psnoise_f(indim, p, per); break;
+                case kUPerlin: v.x = pnoise_f(indim, p, per); break;
+                case kCell: v.x = pcellnoise_f(indim, p, per); break;
+                default: v.x = phashnoise_f(indim, p, per); break;
+                }
+            } else {
+                switch (kind) {
+                case kPerlin: v = psnoise_v(indim, p, per); break;
+                case kUPerlin: v = pnoise_v(indim, p, per); break;
+                case kCell: v = pcellnoise_v(indim, p, per); break;
+                default: v = phashnoise_v(indim, p, per); break;
+                }
+            }
+            const float r[3] = { v.x, v.y, v.z };
+            for (int c = 0; c < outdim; ++c)
+                out[c * n + i] = r[c];
+            if (derivs)
+                for (int c = 0; c < 2 * outdim; ++c)
+                    out[(outdim + c) * n + i] = 0.0f;
+        } else {
+            D2<float> p[4];
+            for (int c = 0; c < indim; ++c)
+                p[c] = D2<float>(in[c * n + i], in[(indim + c) * n + i],
+                                 in[(2 * indim + c) * n + i]);
+            if (outdim == 1) {
+                D2<float> r = kind == kPerlin ? psnoise_df(indim, p, per) : pnoise_df(indim, p, per);
+                out[i]         = r.v;
+                out[n + i]     = r.dx;
+                out[2 * n + i] = r.dy;
+            } else {
+                D2<V3> r = kind == kPerlin ? psnoise_dv(indim, p, per) : pnoise_dv(indim, p, per);
+                const float vals[9] = { r.v.x,  r.v.y,  r.v.z,  r.dx.x, r.dx.y,
+                                        r.dx.z, r.dy.x, r.dy.y, r.dy.z };
+                for (int c = 0; c < 9; ++c)
+                    out[c * n + i] = vals[c];
+            }
+        }
+    }
+    return 0;
+}
+
 int oref_hash_op(int indim, const float* in, int32_t* out, int64_t n)
 {
     if (indim < 1 || indim > 4)

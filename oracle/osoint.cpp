@@ -400,33 +400,69 @@ struct Exec {
             }
     }
 
+    // noise/snoise/cellnoise/hashnoise and the periodic pnoise/psnoise,
+    // with or without a name; the periodic forms take (S, Sper) or
+    // (S, T, Sper, Tper) (batched_llvm_gen.cpp:6066-6081).
     void noise(int L, const Op& op)
     {
-        const Master& m = *G.layers[L].m;
-        std::string name = op.name == "snoise"      ? "perlin"
-                           : op.name == "cellnoise" ? "cell"
-                           : op.name == "hashnoise" ? "hash"
-                                                    : "uperlin";
+        const Master& m     = *G.layers[L].m;
+        const bool periodic = op.name == "pnoise" || op.name == "psnoise";
+        std::string name    = (op.name == "snoise" || op.name == "psnoise") ? "perlin"
+                              : op.name == "cellnoise"                      ? "cell"
+                              : op.name == "hashnoise"                      ? "hash"
+                                                                            : "uperlin";
         size_t first = 1;
         if (op.args.size() > 1 && m.syms[op.args[1]].base == kString) {
             name  = Sv(L, op.args[1]);
             first = 2;
         }
-        float c[4];
-        int dim = 0;
-        size_t a = first;
-        for (; a < op.args.size() && B(L, op.args[a]) != kString; ++a) {
-            if (B(L, op.args[a]) == kTriple) {
-                V3 v     = T(L, op.args[a]);
-                c[dim++] = v.x;
-                c[dim++] = v.y;
-                c[dim++] = v.z;
+        size_t nnum = 0;
+        while (first + nnum < op.args.size() && B(L, op.args[first + nnum]) != kString)
+            ++nnum;
+        const size_t ncoord = periodic ? nnum / 2 : nnum;
+        float c[4], per[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+        int dim = 0, nper = 0;
+        for (size_t k = 0; k < nnum; ++k) {
+            const int sym = op.args[first + k];
+            float* dst    = k < ncoord ? c : per;
+            int& cnt      = k < ncoord ? dim : nper;
+            if (B(L, sym) == kTriple) {
+                V3 v       = T(L, sym);
+                dst[cnt++] = v.x;
+                dst[cnt++] = v.y;
+                dst[cnt++] = v.z;
             } else {
-                c[dim++] = F(L, op.args[a]);
+                dst[cnt++] = F(L, sym);
             }
         }
+        size_t a      = first + nnum;
         const int r   = op.args[0];
         const bool tr = B(L, r) == kTriple;
+        if (periodic && name != "gabor") {
+            std::string pn = (name == "perlin" || name == "snoise" || name == "psnoise")  ? "psnoise"
+                             : (name == "uperlin" || name == "noise" || name == "pnoise") ? "pnoise"
+                             : (name == "cell" || name == "cellnoise")                    ? "cell"
+                             : (name == "hash" || name == "hashnoise")                    ? "hash"
+                                                                                          : "";
+            if (pn.empty() || nper != dim) {
+                fprintf(stderr, "osoint: periodic noise '%s'\n", name.c_str());
+                abort();
+            }
+            if (pn == "psnoise") {
+                if (tr) setT(L, r, psnoise_v(dim, c, per));
+                else setF(L, r, psnoise_f(dim, c, per));
+            } else if (pn == "pnoise") {
+                if (tr) setT(L, r, pnoise_v(dim, c, per));
+                else setF(L, r, pnoise_f(dim, c, per));
+            } else if (pn == "cell") {
+                if (tr) setT(L, r, pcellnoise_v(dim, c, per));
+                else setF(L, r, pcellnoise_f(dim, c, per));
+            } else {
+                if (tr) setT(L, r, phashnoise_v(dim, c, per));
+                else setF(L, r, phashnoise_f(dim, c, per));
+            }
+            return;
+        }
         if (name == "perlin" || name == "snoise") {
             if (tr) setT(L, r, snoise_v(dim, c));
             else setF(L, r, snoise_f(dim, c));
@@ -461,7 +497,10 @@ struct Exec {
             }
             D2<float> p[3] = { D2<float>(c[0]), D2<float>(dim > 1 ? c[1] : float(0.0f)),
                                D2<float>(dim > 2 ? c[2] : float(0.0f)) };
-            setF(L, r, gabor(dim, p, go).v);
+            const int gd = dim < 3 ? dim : 3;  // a 4th (time) coordinate is ignored
+            const float* pp = periodic ? per : nullptr;
+            if (tr) setT(L, r, gabor3(gd, p, go, pp).v);
+            else setF(L, r, gabor_ex(gd, p, go, 0, pp).v);
         } else {
             fprintf(stderr, "osoint: noise '%s'\n", name.c_str());
             abort();
@@ -652,7 +691,8 @@ struct Exec {
             else setF(L, r, (float)v);
             return;
         }
-        if (o == "noise" || o == "snoise" || o == "cellnoise" || o == "hashnoise") {
+        if (o == "noise" || o == "snoise" || o == "cellnoise" || o == "hashnoise" || o == "pnoise"
+            || o == "psnoise") {
             noise(L, op);
             return;
         }

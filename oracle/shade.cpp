@@ -215,6 +215,44 @@ int oref_gabor(int dim, int anisotropic, int do_filter, const float* direction, 
     return 0;
 }
 
+// Gabor noise with a result of outdim 1 or 3 components (gabor / gabor3)
+// and optional periods (pgabor / pgabor3; dim planes of n floats, or null).
+// Out: 3*outdim planes (values, then dx, then dy).
+int oref_gabor_ex(int dim, int outdim, int anisotropic, int do_filter, const float* direction,
+                  float bandwidth, float impulses, const float* in, const float* period,
+                  float* out, int64_t n)
+{
+    if (dim < 1 || dim > 3 || (outdim != 1 && outdim != 3))
+        return -1;
+    GaborOpts o;
+    o.anisotropic = anisotropic;
+    o.do_filter   = do_filter;
+    o.direction   = V3(direction[0], direction[1], direction[2]);
+    o.bandwidth   = bandwidth;
+    o.impulses    = impulses;
+    for (int64_t i = 0; i < n; ++i) {
+        D2<float> p[3];
+        float per[3] = { 0.0f, 0.0f, 0.0f };
+        for (int c = 0; c < dim; ++c) {
+            p[c] = D2<float>(in[c * n + i], in[(dim + c) * n + i], in[(2 * dim + c) * n + i]);
+            if (period)
+                per[c] = period[c * n + i];
+        }
+        if (outdim == 1) {
+            D2<float> r    = gabor_ex(dim, p, o, 0, period ? per : nullptr);
+            out[i]         = r.v;
+            out[n + i]     = r.dx;
+            out[2 * n + i] = r.dy;
+        } else {
+            D2<V3> r = gabor3(dim, p, o, period ? per : nullptr);
+            const float vals[9] = { r.v.x, r.v.y, r.v.z, r.dx.x, r.dx.y, r.dx.z, r.dy.x, r.dy.y, r.dy.z };
+            for (int c = 0; c < 9; ++c)
+                out[c * n + i] = vals[c];
+        }
+    }
+    return 0;
+}
+
 void oref_blackbody(const float* T, float* rgb, int64_t n)
 {
     ColorSys cs;

@@ -58,6 +58,24 @@ def noise(name, outdim, inp, derivs=False):
     return out
 
 
+def pnoise(name, outdim, inp, period, derivs=False):
+    """Periodic noise. inp as for noise(); period: float32 [indim, n]
+    (one period per input dimension and point)."""
+    L = lib()
+    L.oref_pnoise.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    L.oref_pnoise.restype = ctypes.c_int
+    inp = np.ascontiguousarray(inp, dtype=np.float32)
+    planes, n = inp.shape
+    indim = planes // 3 if derivs else planes
+    per = np.ascontiguousarray(np.broadcast_to(np.asarray(period, np.float32), (indim, n)))
+    out = np.zeros(((3 if derivs else 1) * outdim, n), dtype=np.float32)
+    rc = L.oref_pnoise(name.encode(), outdim, indim, int(derivs), _p(inp), _p(per), _p(out), n)
+    if rc != 0:
+        raise ValueError("oracle rejected periodic noise request %r" % name)
+    return out
+
+
 def perlin_cgdefault(outdim, inp, derivs=False):
     """Signed perlin in the CGDefault (SIMD) interpolation order."""
     L = lib()
@@ -101,6 +119,28 @@ def gabor(inp, anisotropic=0, do_filter=1, direction=(1.0, 0.0, 0.0), bandwidth=
     d = np.asarray(direction, np.float32)
     assert L.oref_gabor(dim, anisotropic, do_filter, _p(d), bandwidth, impulses, _p(inp), _p(out),
                         inp.shape[1]) == 0
+    return out
+
+
+def gabor_ex(inp, outdim=1, period=None, anisotropic=0, do_filter=1, direction=(1.0, 0.0, 0.0),
+             bandwidth=1.0, impulses=16.0):
+    """Gabor noise of 1 or 3 components (gabor / gabor3), periodic when
+    period ([dim, n] or broadcastable) is given. inp: [3*dim, n] Dual2
+    planes. Returns [3*outdim, n] (values, dx, dy)."""
+    L = _setup_more()
+    P = ctypes.c_void_p
+    L.oref_gabor_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
+                                ctypes.c_float, ctypes.c_float, P, P, P, ctypes.c_int64]
+    L.oref_gabor_ex.restype = ctypes.c_int
+    inp = np.ascontiguousarray(inp, dtype=np.float32)
+    dim, n = inp.shape[0] // 3, inp.shape[1]
+    per = None
+    if period is not None:
+        per = np.ascontiguousarray(np.broadcast_to(np.asarray(period, np.float32), (dim, n)))
+    out = np.zeros((3 * outdim, n), np.float32)
+    d = np.asarray(direction, np.float32)
+    assert L.oref_gabor_ex(dim, outdim, anisotropic, do_filter, _p(d), bandwidth, impulses,
+                           _p(inp), None if per is None else _p(per), _p(out), n) == 0
     return out
 
 
@@ -471,17 +511,6 @@ def group_run_ex(group, P, N, wo, dPdu=None, dPdv=None, wi=None, rnd=None, nthre
     return out, ev, samp
 
 
-def group_run_sample(group, P, N, wo, rnd, nthreads=1):
-    """The interpreter group's outputs and Ci sampled at (wo, rnd)."""
-    L = group.L
-    L.oref_group_run_sample.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_void_p,
-                                                               ctypes.c_void_p, ctypes.c_int]
-    a = [np.ascontiguousarray(x, np.float32) for x in (P, N, wo, rnd)]
-    n = a[0].shape[1]
-    out = np.zeros((4, n), np.float32)
-    samp = np.zeros((7, n), np.float32)
-    L.oref_group_run_sample(group.h, *[_p(x) for x in a], n, _p(out), _p(samp), nthreads)
-    return out, samp
 def group_closures(group, P, N, wo, dPdu=None, dPdv=None, max_lobes=8):
     """Ci's lobes in the closure-export layout: [1 + 25 max_lobes, n]."""
     L = group.L
@@ -496,3 +525,14 @@ def group_closures(group, P, N, wo, dPdu=None, dPdv=None, max_lobes=8):
     return out
 
 
+def group_run_sample(group, P, N, wo, rnd, nthreads=1):
+    """The interpreter group's outputs and Ci sampled at (wo, rnd)."""
+    L = group.L
+    L.oref_group_run_sample.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_void_p,
+                                                               ctypes.c_void_p, ctypes.c_int]
+    a = [np.ascontiguousarray(x, np.float32) for x in (P, N, wo, rnd)]
+    n = a[0].shape[1]
+    out = np.zeros((4, n), np.float32)
+    samp = np.zeros((7, n), np.float32)
+    L.oref_group_run_sample(group.h, *[_p(x) for x in a], n, _p(out), _p(samp), nthreads)
+    return out, samp

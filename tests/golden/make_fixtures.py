@@ -149,7 +149,8 @@ def main():
         json.dump(tc, f, indent=1)
     imgs = {}
     for name in ("noise", "cellnoise", "hashnoise", "noise-simplex", "noise-gabor",
-                 "noise-gabor2d-filter", "pnoise-gabor"):
+                 "noise-gabor2d-filter", "pnoise-gabor", "pnoise", "pnoise-perlin", "pnoise-cell",
+                 "pnoise-generic"):
         imgs[name.replace("-", "_")] = read_tif("testsuite/%s/ref/out.tif" % name)
     bb, chans = read_exr_half("testsuite/blackbody/ref/out.exr")
     assert chans == ["B", "G", "R"]

@@ -5,12 +5,14 @@
 // blockDim = gridDim = 1). Reads n, then 3-plane P, N, dPdu, dPdv, wo, wi
 // (float32), and the libbsdl table file; writes the 4 eval planes, or with
 // a fourth argument "export" the closure export (1 + 8 records of
-// OSLMI_CLOSURE_RECORD planes) of a kernel built with it.
+// OSLMI_CLOSURE_RECORD planes) of a kernel built with it, or with "out<k>"
+// the group's first k output planes.
 //
-// usage: harness <input> <output> <bsdl_luts.bin> [export]
+// usage: harness <input> <output> <bsdl_luts.bin> [export | out<k>]
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -70,7 +72,10 @@ int main(int argc, char** argv)
 {
     if (argc != 4 && argc != 5)
         return 1;
-    const bool xp = argc == 5;
+    const bool xp   = argc == 5 && strcmp(argv[4], "export") == 0;
+    const int nouts = (argc == 5 && strncmp(argv[4], "out", 3) == 0) ? atoi(argv[4] + 3) : 0;
+    if (argc == 5 && !xp && (nouts < 1 || nouts > OSLMI_MAX_OUTPUT_PLANES))
+        return 1;
     FILE* f = fopen(argv[1], "rb");
     int64_t n = 0;
     if (!f || fread(&n, 8, 1, f) != 1 || n < 1)
@@ -95,8 +100,12 @@ int main(int argc, char** argv)
     A.g[oslmi::kG_dPdv] = dv.data();
     A.wo                = wo.data();
     A.bsdl              = lut.data();
-    std::vector<float> clo;
-    if (xp) {
+    std::vector<float> clo, outs;
+    if (nouts) {
+        outs.assign((size_t)nouts * n, 0.0f);
+        for (int j = 0; j < nouts; ++j)
+            A.out[j] = outs.data() + (size_t)j * n;
+    } else if (xp) {
         clo.assign((size_t)(1 + 8 * OSLMI_CLOSURE_RECORD) * n, 0.0f);
         A.clo     = clo.data();
         A.clo_max = 8;
@@ -106,7 +115,7 @@ int main(int argc, char** argv)
     }
     oslmi_group_main(A);
     FILE* o = fopen(argv[2], "wb");
-    const std::vector<float>& res = xp ? clo : ev;
+    const std::vector<float>& res = nouts ? outs : xp ? clo : ev;
     if (!o || fwrite(res.data(), 4, res.size(), o) != res.size())
         return 5;
     fclose(o);

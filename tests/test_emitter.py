@@ -256,3 +256,32 @@ def test_materialx_closures_emit_and_reject_what_they_do_not_model():
     # classic groups are compiled without the MaterialX lobe state
     ss2, g2 = S.build_layered3()
     assert "OSLMI_MX_CLOSURES" not in ss2.emit(g2)
+
+
+def test_periodic_noise_emits_lattice_wraps_and_rejects_simplex():
+    """pnoise/psnoise lower to the wrapped lattice (make_wrap for perlin,
+    wrap_period for cell/hash and gabor's cell seeds), the vector gabor to
+    gabor3; periodic simplex and a period count that does not match the
+    coordinates are rejected, as the reference rejects them."""
+    import _pnprogs as PN
+    ss = S.ShadingSystem(jit_cache=None)
+    want = {"pn_perlin": "make_wrap(", "pn_uperlin": "make_wrap(", "pn_default": "make_wrap(",
+            "ps_default": "make_wrap(", "pn_cell": "wrap_period(", "pn_hash": "wrap_period(",
+            "pn_gabor": "gabor3(gu_", "n_gabor3": "gabor3(gu_"}
+    for name, text in PN.programs():
+        g = S.build_generated_group(ss, ([(name, "surface", text)], []), name, outputs=PN.OUTPUTS)
+        src = ss.emit(g)
+        assert want[name] in src, name
+        if name == "pn_gabor":
+            assert ", true, V3(" in src
+    hdr = ("OpenShadingLanguage 1.00\nshader s\nglobal point P\noparam float f 0\n"
+           "const string $c1 \"simplex\"\nconst float $c2 3\n")
+    for body, word in (("\tpnoise f $c1 P P\n", "periodic noise type"),
+                       ("\tpnoise f P\n", "Pperiod"),
+                       ("\tpnoise f P $c2\n", "periods do not match")):
+        ss.LoadMemoryCompiledShader("s", hdr + "code ___main___\n" + body + "\tend\n")
+        g = ss.ShaderGroupBegin("g")
+        ss.Shader(g, "surface", "s", "a")
+        ss.ShaderGroupEnd(g)
+        with pytest.raises(OSLMIError, match=word):
+            ss.emit(g)

@@ -5,6 +5,8 @@ function of the integer hash) bit-exact; float noise within 4 ulp or 1e-5
 relative, whichever is larger (both paths evaluate the same IEEE operations
 in the same order, so in practice they agree exactly).
 """
+import zlib
+
 import numpy as np
 import pytest
 
@@ -38,6 +40,10 @@ def _inputs(indim, n, derivs, seed):
     return p
 
 
+def _seed(*key):
+    return zlib.crc32(repr(key).encode()) & 0xffff
+
+
 @pytest.fixture(scope="module")
 def dev():
     from openshadinglanguage_amd import check, lib
@@ -53,7 +59,7 @@ CASES = [(nm, od, idim, dv) for nm in ("perlin", "uperlin", "cell", "hash", "sim
 def test_noise_parity(dev, name, outdim, indim, derivs):
     from openshadinglanguage_amd import shadeops
     n = 4099  # ragged: not a multiple of the 256-thread block
-    inp = _inputs(indim, n, derivs, seed=hash((name, outdim, indim, derivs)) & 0xffff)
+    inp = _inputs(indim, n, derivs, seed=_seed(name, outdim, indim, derivs))
     ref = O.noise(name, outdim, inp, derivs=derivs)
     got = shadeops.noise(name, outdim, torch.from_numpy(inp).to(dev), derivs=derivs).cpu().numpy()
     assert got.shape == ref.shape
@@ -77,3 +83,41 @@ def test_empty_batch(dev):
     from openshadinglanguage_amd import shadeops
     out = shadeops.noise("perlin", 1, torch.zeros((3, 0), device=dev))
     assert out.shape == (1, 0)
+
+
+PCASES = [(nm, od, idim, dv) for nm in ("perlin", "uperlin", "cell", "hash")
+          for od in (1, 3) for idim in (1, 2, 3, 4) for dv in (False, True)]
+
+
+@pytest.mark.parametrize("name,outdim,indim,derivs", PCASES)
+def test_pnoise_parity(dev, name, outdim, indim, derivs):
+    """Periodic noise (oslmi_pnoise) against the oracle. Periods per point
+    and axis: whole and fractional values, values below 1 (taken as 1) and,
+    for perlin, NaN (lattice period 1); coordinates of both signs so the
+    negative remainders of imod are exercised."""
+    from openshadinglanguage_amd import shadeops
+    n = 4099
+    seed = _seed("p", name, outdim, indim, derivs)
+    inp = _inputs(indim, n, derivs, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    per = rng.choice(np.array([0.25, 1.0, 2.0, 3.0, 3.7, 5.0, 8.0, 13.5, -2.0], np.float32),
+                     size=(indim, n))
+    if name in ("perlin", "uperlin"):
+        per[:, 100:140] = np.nan
+    ref = O.pnoise(name, outdim, inp, per, derivs=derivs)
+    got = shadeops.pnoise(name, outdim, torch.from_numpy(inp).to(dev),
+                          torch.from_numpy(per).to(dev), derivs=derivs).cpu().numpy()
+    assert got.shape == ref.shape
+    if name in ("cell", "hash"):
+        np.testing.assert_array_equal(got.view(np.int32), ref.view(np.int32))
+    else:
+        ok, ulp = _ulp_close(got, ref)
+        assert ok.all(), "max ulp %d at %s" % (ulp.max(), np.argwhere(~ok)[:4])
+
+
+def test_pnoise_rejects_simplex(dev):
+    from openshadinglanguage_amd import lib, shadeops
+    x = torch.zeros((1, 4), device=dev)
+    with pytest.raises(Exception):
+        shadeops.pnoise("simplex", 1, x, torch.ones((1, 4), device=dev))
+    assert b"unknown" in lib().oslmi_last_error()

@@ -321,3 +321,185 @@ def test_cgdefault_variant_is_bounded(dim, outdim, derivs):
     if derivs:
         assert d[outdim:].max() <= 1.2e-6
     assert 0.2 < np.mean(a == b) < 0.6  # the two orders really differ
+
+
+# --- periodic noise ----------------------------------------------------------
+
+def _testpnoise_coords(res=512, inscale=30.0, ttime_from_u=False):
+    """Coordinates of testsuite/common/shaders/testpnoise.osl (angle 0:
+    cos = 1, sin = 0 exactly) and of testsuite/pnoise/test.osl (inscale 40,
+    ttime = u * inscale), testshade -g res res without -center."""
+    f = np.float32
+    x = np.arange(res, dtype=f)
+    u = (x / f(res - 1)).astype(f)
+    U, V = np.meshgrid(u, u)
+    s = (U - f(0.25) * np.floor(U / f(0.25))) - f(0.125)
+    t = (V - f(0.5) * np.floor(V / f(0.5))) - f(0.25)
+    ru = ((f(1) * s + f(0) * t) * f(inscale)).astype(f)
+    rv = ((f(0) * s - f(1) * t) * f(inscale)).astype(f)
+    rz = ((f(inscale) * (s + t)) / f(np.sqrt(2.0))).astype(f)
+    tt = ((U if ttime_from_u else s) * f(inscale)).astype(f)
+    return U, V, ru, rv, rz, tt
+
+
+def _pcols(U, V):
+    cols = {1: U < 0.24, 2: (U > 0.26) & (U < 0.49), 3: (U > 0.51) & (U < 0.74), 4: U > 0.76}
+    return [(dim, outdim, cols[dim] & rows) for dim in (1, 2, 3, 4)
+            for outdim, rows in ((1, V < 0.49), (3, V > 0.51))]
+
+
+def _render_periodic(name, offset, scale, inscale, ttime_from_u, period):
+    f = np.float32
+    U, V, ru, rv, rz, tt = _testpnoise_coords(512, inscale, ttime_from_u)
+    out = np.zeros(U.shape + (3,), f)
+    for dim, outdim, m in _pcols(U, V):
+        inp = np.stack([ru[m], rv[m], rz[m], tt[m]][:dim] if dim != 2 else [ru[m], rv[m]])
+        r = O.pnoise(name, outdim, inp, np.full((dim, 1), period, f))
+        if outdim == 1:
+            r = np.repeat(r, 3, axis=0)
+        out[m] = (f(offset) + f(scale) * r).T
+    return out
+
+
+# (fixture, noise name, offset, scale, inscale, ttime from u): run.py of each
+# test; test.osl of testsuite/pnoise calls pnoise() without a name
+PERIODIC_IMAGES = [
+    ("pnoise", "pnoise", 0.0, 1.0, 40.0, True),
+    ("pnoise_perlin", "perlin", 0.5, 0.5, 30.0, False),
+    ("pnoise_cell", "cell", 0.0, 1.0, 30.0, False),
+]
+
+
+@pytest.mark.parametrize("image,name,offset,scale,inscale,ttime_from_u", PERIODIC_IMAGES)
+def test_periodic_reference_images(image, name, offset, scale, inscale, ttime_from_u):
+    """testsuite/pnoise, pnoise-perlin and pnoise-cell (period 3 on every
+    axis, float rows and color rows in 1-4 dimensions) reproduce exactly;
+    with period 4 or no wrap at all (1e6) most pixels change, so the images
+    pin the imod of the lattice hash and the float wrap of cell noise."""
+    ref = np.load(os.path.join(HERE, "golden", "noise_images.npz"))[image].astype(np.int32)
+    diff = np.abs(_to_u8(_render_periodic(name, offset, scale, inscale, ttime_from_u, 3.0)) - ref)
+    assert diff.max() == 0, (diff.max(), np.mean(diff > 0))  # measured: exact
+    for period in (4.0, 1e6):
+        other = np.abs(_to_u8(_render_periodic(name, offset, scale, inscale, ttime_from_u,
+                                                period)) - ref)
+        assert np.mean(other > 1) > 0.5
+
+
+def test_periodic_gabor_reference_image():
+    """testsuite/pnoise-gabor: testpnoise.osl with noisename gabor (period 3,
+    isotropic, filtered), both the float rows and the color rows (gabor3:
+    seeds 0, 1, 2 per channel). The 4D column ignores the time coordinate
+    as the reference does (pgabor(p, pp))."""
+    f = np.float32
+    res, inscale = 512, 30.0
+    U, V, ru, rv, rz, _ = _testpnoise_coords(res, inscale)
+    dud = f(1.0) / f(res - 1)
+    dx = [(f(1) * dud) * f(inscale), (f(0) * dud) * f(inscale), (f(inscale) * dud) / f(np.sqrt(2))]
+    dy = [(f(0) * dud) * f(inscale), (-(f(1) * dud)) * f(inscale), (f(inscale) * dud) / f(np.sqrt(2))]
+    ref = np.load(os.path.join(HERE, "golden", "noise_images.npz"))["pnoise_gabor"].astype(np.int32)
+    for dim, outdim, m in _pcols(U, V):
+        k, gd = int(m.sum()), min(dim, 3)
+        vals = [ru[m], rv[m], rz[m]][:gd]
+        inp = np.stack(vals + [np.full(k, q, f) for q in dx[:gd]] + [np.full(k, q, f) for q in dy[:gd]])
+        r = O.gabor_ex(inp, outdim, period=np.full((gd, 1), 3.0, f))
+        got = _to_u8(f(0.5) + f(0.5) * r[:outdim])
+        if outdim == 1:
+            got = np.repeat(got, 3, 0)
+        diff = np.abs(got.T - ref[m])
+        # measured: at most 1 LSB on under 1e-4 of the pixels of a block
+        assert diff.max() <= 1 and np.mean(diff > 0) < 2e-4, (dim, outdim, diff.max())
+
+
+def test_gabor3_components_are_the_seeded_scalar():
+    """gabor3's channel c is the scalar evaluation seeded with c; channel 0
+    is the scalar gabor itself."""
+    rng = np.random.default_rng(3)
+    n = 400
+    inp = np.concatenate([rng.uniform(-5, 5, (3, n)), rng.uniform(-0.01, 0.01, (6, n))]).astype(np.float32)
+    v = O.gabor_ex(inp, 3)
+    s = O.gabor(inp)
+    np.testing.assert_array_equal(v[[0, 3, 6]], s)
+    assert np.mean(v[0] == v[1]) < 0.01 and np.mean(v[1] == v[2]) < 0.01
+
+
+@pytest.mark.parametrize("name", ["perlin", "uperlin", "cell", "hash"])
+@pytest.mark.parametrize("dim", [1, 2, 3, 4])
+def test_periodic_noise_is_periodic(name, dim):
+    """Shifting any coordinate by a whole number of (floored) periods leaves
+    the value unchanged; periods below 1 act as 1 and a period's
+    fraction is dropped. Coordinates are multiples of 1/64, so the shifted
+    coordinates and their fractions are exact in float."""
+    rng = np.random.default_rng(dim + 7 * len(name))
+    n = 3000
+    per = rng.integers(1, 9, (dim, n)).astype(np.float32)
+    if name in ("cell", "hash"):
+        p = rng.integers(-40, 40, (dim, n)).astype(np.float32) + (
+            np.float32(0.5) if name == "cell" else np.float32(0.0))
+    else:
+        p = (rng.integers(-40 * 64, 40 * 64, (dim, n)) / 64.0).astype(np.float32)
+    base = O.pnoise(name, 3, p, per)
+    for axis in range(dim):
+        q = p.copy()
+        q[axis] += per[axis] * rng.integers(-3, 4, n).astype(np.float32)
+        np.testing.assert_array_equal(O.pnoise(name, 3, q, per), base)
+    np.testing.assert_array_equal(O.pnoise(name, 3, p, per + np.float32(0.75)), base)
+    one = O.pnoise(name, 1, p, np.ones_like(per))
+    np.testing.assert_array_equal(O.pnoise(name, 1, p, np.full_like(per, -2.5)), one)
+    if name in ("perlin", "uperlin"):
+        # the lattice period of NaN is 1 (ifloor gives INT_MIN, clamped);
+        # cell/hash wrap with a NaN period yields NaN coordinates instead
+        np.testing.assert_array_equal(O.pnoise(name, 1, p, np.full_like(per, np.nan)), one)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3, 4])
+def test_periodic_perlin_equals_plain_perlin_inside_one_period(dim):
+    """Inside [0, period) on every axis the lattice corners other than the
+    last never wrap: with a period larger than the whole domain the
+    periodic noise is the plain noise, values and derivatives."""
+    rng = np.random.default_rng(40 + dim)
+    n = 3000
+    p = rng.uniform(0, 50, (dim, n)).astype(np.float32)
+    inp = np.concatenate([p, rng.uniform(-1, 1, (2 * dim, n)).astype(np.float32)])
+    big = np.full((dim, 1), 1000.0, np.float32)
+    for name in ("perlin", "uperlin"):
+        for outdim in (1, 3):
+            np.testing.assert_array_equal(O.pnoise(name, outdim, inp, big, derivs=True),
+                                          O.noise(name, outdim, inp, derivs=True))
+    np.testing.assert_array_equal(O.pnoise("cell", 3, p, big), O.noise("cell", 3, p))
+
+
+@pytest.mark.parametrize("prog", ["pn_perlin", "pn_uperlin", "pn_cell", "pn_hash", "pn_default",
+                                  "ps_default", "pn_gabor", "n_gabor3"])
+def test_interpreter_noise_ops_split_coordinates_and_periods(prog):
+    """The interpreter's pnoise/psnoise ops read (S, Sper) / (S, T, Sper,
+    Tper) as llvm_gen_noise does (batched_llvm_gen.cpp:6066-6081), the
+    unnamed forms are the unsigned / signed periodic perlin, and gabor
+    ignores a 4th (time) coordinate: each output equals the direct call."""
+    import _pnprogs as PN
+    spec = {p: (op, nm) for p, op, nm in PN.SPECS}[prog]
+    text = dict(PN.programs())[prog]
+    inp = PN.inputs(300, 5)
+    n = 300
+    g = O.Group([(prog, "surface", text)], [])
+    f = np.float32
+    coords = {1: [inp["P"][0]], 2: [inp["P"][0], inp["P"][1]], 3: list(inp["P"]),
+              4: list(inp["P"]) + [inp["dPdu"][0]]}
+    pers = {1: [inp["N"][0]], 2: [inp["N"][0], inp["N"][1]], 3: list(inp["N"]),
+            4: list(inp["N"]) + [inp["dPdv"][0]]}
+    name = spec[1] or ("uperlin" if spec[0] == "pnoise" else "perlin")
+    for d in (1, 2, 3, 4):
+        g.outputs("F%d" % d, "C%d" % d)
+        got, _, _ = O.group_run_ex(g, inp["P"], inp["N"], np.zeros((3, n), f), dPdu=inp["dPdu"],
+                                   dPdv=inp["dPdv"])
+        c, p = np.stack(coords[d]).astype(f), np.stack(pers[d]).astype(f)
+        if name == "gabor":
+            gd = min(d, 3)
+            dual = np.concatenate([c[:gd], np.zeros((2 * gd, n), f)])
+            per = p[:gd] if spec[0] == "pnoise" else None
+            want_f = O.gabor_ex(dual, 1, period=per)[0]
+            want_c = O.gabor_ex(dual, 3, period=per)[:3]
+        else:
+            want_f = O.pnoise(name, 1, c, p)[0]
+            want_c = O.pnoise(name, 3, c, p)
+        np.testing.assert_array_equal(got[0], want_f)
+        np.testing.assert_array_equal(got[1:4], want_c)
