@@ -38,6 +38,7 @@
 #include "oslmi_color.h"
 #include "oslmi_group.h"
 #include "oslmi_internal.h"
+#include "oslmi_tiles.h"
 
 namespace oslmi {
 namespace {
@@ -180,40 +181,27 @@ struct Batch {
     }
 
     // The tile list of items with `counts` points under the current config
-    // (no device work): tiles of block * ppt points, each part's items
-    // together, in item order or by measured cost per tile.
+    // (no device work; oslmi_tiles.h).
     int tile_list(const std::vector<int64_t>& counts, std::vector<BatchTile>& out,
                   std::vector<int>& first, std::vector<int>& count, std::vector<int>& part_first,
                   std::vector<int>& part_ntiles) const
     {
-        const int64_t ts = (int64_t)cfg.block * cfg.ppt;
-        const int ni     = (int)counts.size();
-        out.clear();
-        first.assign(ni, 0);
-        count.assign(ni, 0);
-        part_first.assign(parts.size(), 0);
-        part_ntiles.assign(parts.size(), 0);
-        for (size_t q = 0; q < parts.size(); ++q) {
-            std::vector<int> order = parts[q].items;
-            if (cfg.order == 1 && (int)cost.size() == ni)
-                std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-            part_first[q] = (int)out.size();
-            for (int k : order) {
-                first[k] = (int)out.size();
-                for (int64_t b = 0; b < counts[k]; b += ts) {
-                    BatchTile t;
-                    t.item  = k;
-                    t.pad   = 0;
-                    t.begin = b;
-                    t.end   = std::min<int64_t>(b + ts, counts[k]);
-                    out.push_back(t);
-                    if (out.size() > (size_t)0x7fffffff)
-                        return set_error(OSLMI_ERR_INVALID, "oslmi_batch: too many tiles");
-                }
-                count[k] = (int)out.size() - first[k];
-            }
-            part_ntiles[q] = (int)out.size() - part_first[q];
-        }
+        std::vector<std::vector<int>> part_items(parts.size());
+        for (size_t q = 0; q < parts.size(); ++q)
+            part_items[q] = parts[q].items;
+        TileLayout lay;
+        lay.tiles.swap(out);
+        const TileListStatus st =
+            oslmi::tile_list(counts, part_items, (int64_t)cfg.block * cfg.ppt, cfg.order == 1, cost, lay);
+        if (st == kTilesTooMany)
+            return set_error(OSLMI_ERR_INVALID, "oslmi_batch: too many tiles");
+        if (st != kTilesOk)
+            return set_error(OSLMI_ERR_INVALID, "oslmi_batch: bad tile list input");
+        out.swap(lay.tiles);
+        first.swap(lay.item_first);
+        count.swap(lay.item_count);
+        part_first.swap(lay.part_first);
+        part_ntiles.swap(lay.part_ntiles);
         return OSLMI_OK;
     }
 
