@@ -48,8 +48,16 @@ struct Lobe {
     int id;
     V3 weight;
     V3 N;
-    float alpha;
+    float alpha;  // microfacet xalpha (and yalpha unless anisotropic)
     float eta;
+#ifdef OSLMI_FULL_MICROFACET
+    // general microfacet lobes: the tangent U of the lobe's frame
+    // (TangentFrame::from_normal_and_tangent), yalpha and the refract mode
+    // (0 reflection, 1 refraction, 2 both)
+    V3 U;
+    float ay;
+    int refract;
+#endif
 #ifdef OSLMI_MX_CLOSURES
     mtx::MxLobe mx;
 #endif
@@ -128,6 +136,34 @@ OSLMI_DEV float fresnel_dielectric(float cosi, float eta)
     return 1.0f;
 }
 
+// fresnel_refraction (src/testrender/optics.h:33-62): the refracted
+// direction T of the incoming direction I (pointing at the surface) about
+// N, and the Fresnel transmittance; 0 and T = 0 on total internal
+// reflection
+OSLMI_DEV float fresnel_refraction(V3 I, V3 N, float eta, V3* T)
+{
+    float cosi = -dot(I, N);
+    V3 Nn;
+    float neta;
+    if (cosi > 0) {
+        neta = 1 / eta;
+        Nn   = N;
+    } else {
+        cosi = -cosi;
+        neta = eta;
+        Nn   = -N;
+    }
+    const float arg = 1.0f - (neta * neta * (1.0f - cosi * cosi));
+    if (arg >= 0) {
+        const float dnp = sqrtf(arg);
+        const float nK  = (neta * cosi) - dnp;
+        *T              = I * neta + Nn * nK;
+        return 1 - fresnel_dielectric(cosi, eta);
+    }
+    *T = V3(0.0f);
+    return 0;
+}
+
 struct Frame {
     V3 u, v, w;
 };
@@ -144,6 +180,56 @@ OSLMI_DEV Frame frame_from_normal(V3 n)
     return f;
 }
 OSLMI_DEV V3 tolocal(const Frame& f, V3 a) { return V3(dot(a, f.u), dot(a, f.v), dot(a, f.w)); }
+// TangentFrame::from_normal_and_tangent (src/testrender/sampling.h:30-42):
+// x = t - n (n.t) normalised, y = n x x; a tangent parallel to n (or
+// zero) falls back to from_normal
+OSLMI_DEV Frame frame_from_normal_tangent(V3 n, V3 t)
+{
+    V3 x              = t - n * dot(n, t);
+    const float xlen2 = dot(x, x);
+    if (xlen2 > 0) {
+        x = x * (1.0f / sqrtf(xlen2));
+        Frame f;
+        f.u = x;
+        f.v = cross(n, x);
+        f.w = n;
+        return f;
+    }
+    return frame_from_normal(n);
+}
+// the frame and yalpha of a microfacet lobe
+OSLMI_DEV Frame lobe_frame(const Lobe& l)
+{
+#ifdef OSLMI_FULL_MICROFACET
+    return frame_from_normal_tangent(l.N, l.U);
+#else
+    return frame_from_normal(l.N);
+#endif
+}
+OSLMI_DEV float lobe_yalpha(const Lobe& l)
+{
+#ifdef OSLMI_FULL_MICROFACET
+    return l.ay;
+#else
+    return l.alpha;
+#endif
+}
+OSLMI_DEV int lobe_refract(const Lobe& l)
+{
+#ifdef OSLMI_FULL_MICROFACET
+    return l.refract;
+#else
+    (void)l;
+    return 0;
+#endif
+}
+// Vec3::normalize as the reference's microfacet code uses it: unchanged
+// when the length is zero
+OSLMI_DEV V3 normalized_or_self(V3 h)
+{
+    const float hl = sqrtf(dot(h, h));
+    return hl != 0.0f ? V3(h.x / hl, h.y / hl, h.z / hl) : h;
+}
 
 OSLMI_DEV float ggx_lambda(V3 w, float xa, float ya)
 {
@@ -213,24 +299,63 @@ OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
         r.pdf    = fmaxf(dot(l.N, wi), 0.0f) * 0.31830987334251403809f;  // float(M_1_PI)
         return r;
     }
-    Frame tf = frame_from_normal(l.N);
-    V3 wo_l  = tolocal(tf, wo);
-    V3 wi_l  = tolocal(tf, wi);
-    if (wo_l.z > 0 && wi_l.z > 0) {
-        V3 h     = wi_l + wo_l;
-        float hl = sqrtf(dot(h, h));
-        V3 m     = hl != 0.0f ? V3(h.x / hl, h.y / hl, h.z / hl) : h;
-        const bool bk = l.id == kClosureMicrofacetBeckmann;
-        float D  = bk ? beckmann_D(m, l.alpha, l.alpha) : ggx_D(m, l.alpha, l.alpha);
-        float Lo = bk ? beckmann_lambda(wo_l, l.alpha, l.alpha) : ggx_lambda(wo_l, l.alpha, l.alpha);
-        float Li = bk ? beckmann_lambda(wi_l, l.alpha, l.alpha) : ggx_lambda(wi_l, l.alpha, l.alpha);
+    Frame tf       = lobe_frame(l);
+    V3 wo_l        = tolocal(tf, wo);
+    V3 wi_l        = tolocal(tf, wi);
+    const float xa = l.alpha, ya = lobe_yalpha(l);
+    const int rf   = lobe_refract(l);
+    const bool bk  = l.id == kClosureMicrofacetBeckmann;
+    if (rf != 1 && wo_l.z > 0 && wi_l.z > 0) {
+        V3 m     = normalized_or_self(wi_l + wo_l);
+        float D  = bk ? beckmann_D(m, xa, ya) : ggx_D(m, xa, ya);
+        float Lo = bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya);
+        float Li = bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya);
         float G2 = 1 / (Lo + Li + 1);
         float G1 = 1 / (Lo + 1);
         float Fr = fresnel_dielectric(dot(m, wo_l), l.eta);
         r.pdf    = (G1 * D * 0.25f) / wo_l.z;
-        r.weight = V3((G2 / G1) * Fr);
+        if (rf == 2) {
+            r.pdf *= Fr;
+            r.weight = V3(G2 / G1);
+        } else {
+            r.weight = V3((G2 / G1) * Fr);
+        }
         return r;
     }
+#ifdef OSLMI_FULL_MICROFACET
+    // transmission (shading.cpp:612-646): the half-vector of the
+    // refraction, Fresnel transmittance, the Jacobian eta^2 |.|/|ht|^2
+    if (rf != 0 && wi_l.z < 0 && wo_l.z > 0.0f) {
+        V3 ht = -(wi_l * l.eta + wo_l);
+        if (l.eta < 1.0f)
+            ht = -ht;
+        const V3 Ht       = normalized_or_self(ht);
+        const float cosHO = dot(Ht, wo_l);
+        const float Ft    = 1.0f - fresnel_dielectric(cosHO, l.eta);
+        if (Ft > 0) {
+            const float cosHI = dot(Ht, wi_l);
+            if (Ht.z <= 0.0f) {
+                r.weight = V3(0.0f);
+                r.pdf    = 0.0f;
+                return r;
+            }
+            const float Dt     = bk ? beckmann_D(Ht, xa, ya) : ggx_D(Ht, xa, ya);
+            const float Lo     = bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya);
+            const float Li     = bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya);
+            const float G2     = 1 / (Lo + Li + 1);
+            const float G1     = 1 / (Lo + 1);
+            const float invHt2 = 1 / dot(ht, ht);
+            r.pdf = (fabsf(cosHI * cosHO) * (l.eta * l.eta) * (G1 * Dt) * invHt2) / wo_l.z;
+            if (rf == 2) {
+                r.pdf *= Ft;
+                r.weight = V3(G2 / G1);
+            } else {
+                r.weight = V3((G2 / G1) * Ft);
+            }
+            return r;
+        }
+    }
+#endif
     r.weight = V3(0.0f);
     r.pdf    = 0.0f;
     return r;
@@ -244,7 +369,12 @@ OSLMI_DEV V3 lobe_albedo(const Lobe& l, V3 wo)
 #endif
     if (l.id == kClosureDiffuse)
         return V3(1.0f);
-    return V3(fresnel_dielectric(dot(l.N, wo), l.eta));
+    // Microfacet::get_albedo (shading.cpp:580-588)
+    const int rf = lobe_refract(l);
+    if (rf == 2)
+        return V3(1.0f);
+    const float fr = fresnel_dielectric(dot(l.N, wo), l.eta);
+    return V3(rf ? 1 - fr : fr);
 }
 
 template<int MAXL> OSLMI_DEV void composite_prepare(LobeList<MAXL>& L, V3 wo)
@@ -357,7 +487,7 @@ OSLMI_DEV V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
               1.0f / mlen);
 }
 
-OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
+OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, float rz)
 {
     BsdfDirSample r;
     r.wi     = V3(0.0f);
@@ -386,23 +516,59 @@ OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
         r.pdf    = cos_theta * 0.31830987334251403809f;
         return r;
     }
-    const Frame tf    = frame_from_normal(l.N);
+    const Frame tf    = lobe_frame(l);
     const V3 wo_l     = tolocal(tf, wo);
     const float cosNO = wo_l.z;
     if (!(cosNO > 0))
         return r;
-    const V3 m        = ggx_sample_micronormal(wo_l, l.alpha, l.alpha, rx, ry);
+    const float xa    = l.alpha, ya = lobe_yalpha(l);
+    const int rf      = lobe_refract(l);
+    const V3 m        = ggx_sample_micronormal(wo_l, xa, ya, rx, ry);
     const float cosMO = dot(m, wo_l);
     const float F     = fresnel_dielectric(cosMO, l.eta);
-    const V3 wi_l     = m * (2.0f * cosMO) - wo_l;
-    const float D     = ggx_D(m, l.alpha, l.alpha);
-    const float Lo    = ggx_lambda(wo_l, l.alpha, l.alpha);
-    const float Li    = ggx_lambda(wi_l, l.alpha, l.alpha);
+    if (rf == 0 || (rf == 2 && rz < F)) {
+        // reflection; rz chooses it by the microfacet's Fresnel term when
+        // the lobe both reflects and refracts
+        const V3 wi_l  = m * (2.0f * cosMO) - wo_l;
+        const float D  = ggx_D(m, xa, ya);
+        const float Lo = ggx_lambda(wo_l, xa, ya);
+        const float Li = ggx_lambda(wi_l, xa, ya);
+        const float G2 = 1 / (Lo + Li + 1);
+        const float G1 = 1 / (Lo + 1);
+        r.wi           = frame_get(tf, wi_l.x, wi_l.y, wi_l.z);
+        r.pdf          = (G1 * D * 0.25f) / cosNO;
+        if (rf == 2) {
+            r.pdf *= F;
+            r.weight = V3(G2 / G1);
+        } else {
+            r.weight = V3(F * (G2 / G1));
+        }
+        return r;
+    }
+#ifdef OSLMI_FULL_MICROFACET
+    // transmission through the sampled microfacet (shading.cpp:685-710)
+    const V3 M        = frame_get(tf, m.x, m.y, m.z);
+    V3 wi;
+    const float Ft    = fresnel_refraction(-wo, M, l.eta, &wi);
+    const V3 wi_l     = tolocal(tf, wi);
+    const float cosHO = dot(m, wo_l);
+    const float cosHI = dot(m, wi_l);
+    const float D     = ggx_D(m, xa, ya);
+    const float Lo    = ggx_lambda(wo_l, xa, ya);
+    const float Li    = ggx_lambda(wi_l, xa, ya);
     const float G2    = 1 / (Lo + Li + 1);
     const float G1    = 1 / (Lo + 1);
-    r.wi              = frame_get(tf, wi_l.x, wi_l.y, wi_l.z);
-    r.pdf             = (G1 * D * 0.25f) / cosNO;
-    r.weight          = V3(F * (G2 / G1));
+    const V3 ht       = -(wi_l * l.eta + wo_l);
+    const float invHt2 = 1.0f / dot(ht, ht);
+    r.wi  = wi;
+    r.pdf = (fabsf(cosHI * cosHO) * (l.eta * l.eta) * (G1 * D) * invHt2) / fabsf(wo_l.z);
+    if (rf == 2) {
+        r.pdf *= Ft;
+        r.weight = V3(G2 / G1);
+    } else {
+        r.weight = V3(Ft * (G2 / G1));
+    }
+#endif
     return r;
 }
 
@@ -415,7 +581,6 @@ template<int MAXL>
 OSLMI_DEV BsdfDirSample composite_sample(const LobeList<MAXL>& L, V3 wo, float rx, float ry,
                                          float rz)
 {
-    (void)rz;
     BsdfDirSample s;
     s.wi        = V3(0.0f);
     s.weight    = V3(0.0f);
@@ -429,7 +594,7 @@ OSLMI_DEV BsdfDirSample composite_sample(const LobeList<MAXL>& L, V3 wo, float r
                 done    = true;
                 float u = (rx - accum) / L.pdf[i];
                 u       = fminf(u, 0.99999994f);
-                s       = lobe_sample(L.l[i], wo, u, ry);
+                s       = lobe_sample(L.l[i], wo, u, ry, rz);
                 const V3 wsc = L.l[i].weight * (1 / L.pdf[i]);
                 s.weight     = s.weight * wsc;
                 s.pdf        = s.pdf * L.pdf[i];

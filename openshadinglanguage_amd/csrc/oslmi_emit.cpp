@@ -75,6 +75,7 @@ struct Emitter {
     std::set<int> globals;
     bool uses_cs = false, uses_bb = false;
     bool uses_mx = false;  // MaterialX closures (libbsdl lobes, energy tables)
+    bool uses_aniso = false;  // microfacet lobes with a tangent, two roughnesses or refraction
 
     Emitter(const ShadingSys& s, Group& gr) : ss(s), g(gr) {}
 
@@ -1251,15 +1252,25 @@ struct Emitter {
                     fail("microfacet closure needs 7 arguments");
                     return;
                 }
+                // the refract mode selects the lobe's type (testrender picks
+                // Microfacet<Dist, 0|1|2> per closure): a constant 0, 1 or 2
                 std::vector<float> refr;
-                numval(L, a(8), refr);
-                if (!refr.empty() && refr[0] != 0.0f) {
-                    fail("refracting microfacet is not supported");
+                const bool refr_const = S(L, a(8)).kind == kConst && numval(L, a(8), refr)
+                                        && !refr.empty();
+                const int refract = refr_const ? (int)refr[0] : -1;
+                if (refract < 0 || refract > 2) {
+                    fail("microfacet refract must be a constant 0, 1 or 2");
                     return;
                 }
-                // isotropic lobes about N only: yalpha must be xalpha (the
-                // same symbol, as oslc writes microfacet(dist, N, alpha,
-                // eta, refract), or equal constants) and U the constant 0
+                if (refract != 0 && dist == "beckmann" && ss.closure_sample) {
+                    fail("sampling a beckmann microfacet is not supported");
+                    return;
+                }
+                // isotropic lobes about N (yalpha the same symbol as xalpha,
+                // as oslc writes microfacet(dist, N, alpha, eta, refract),
+                // or equal constants, and U the constant 0) keep the short
+                // lobe; anything else builds the anisotropic lobe with its
+                // tangent frame
                 auto is_const = [&](int sym) { return S(L, sym).kind == kConst; };
                 std::vector<float> xa, ya, u;
                 const bool same_alpha =
@@ -1269,8 +1280,13 @@ struct Emitter {
                 const bool zero_u = is_const(a(4)) && numval(L, a(4), u)
                                     && std::all_of(u.begin(), u.end(),
                                                    [](float v) { return v == 0.0f; });
-                if (!same_alpha || !zero_u) {
-                    fail("anisotropic microfacet (yalpha != xalpha or a tangent U) is not supported");
+                if (!same_alpha || !zero_u || refract != 0) {
+                    uses_aniso = true;
+                    line(R + " = closure_microfacet_full("
+                         + std::string(dist == "ggx" ? "kClosureMicrofacetGGX" : "kClosureMicrofacetBeckmann")
+                         + ", " + ex(L, a(3), K_V3) + ", " + ex(L, a(4), K_V3) + ", " + ex(L, a(5), K_FLOAT)
+                         + ", " + ex(L, a(6), K_FLOAT) + ", " + ex(L, a(7), K_FLOAT) + ", "
+                         + std::to_string(refract) + ");");
                     return;
                 }
                 line(R + " = closure_microfacet_" + dist + "(" + ex(L, a(3), K_V3) + ", " + ex(L, a(5), K_FLOAT)
@@ -1560,7 +1576,7 @@ struct Emitter {
             return false;
         std::ostringstream k;
         k << "// Emitted by the oslmi group emitter for group '" << g.name << "'.\n";
-        k << prelude(uses_mx, g.can_export);
+        k << prelude(uses_mx, g.can_export, uses_aniso);
         k << "extern \"C\" __global__ " << kernel_attributes(ss) << " void oslmi_group_main(GroupArgs A)\n{\n";
         k << "    using Closure = LobeList<" << closure_capacity() << ">;\n";
         if (uses_cs)
@@ -1578,9 +1594,10 @@ struct Emitter {
         return true;
     }
 
-    static std::string prelude(bool mx, bool xp = false)
+    static std::string prelude(bool mx, bool xp = false, bool aniso = false)
     {
         return std::string(mx ? "#define OSLMI_MX_CLOSURES 1\n" : "")
+               + std::string(aniso ? "#define OSLMI_FULL_MICROFACET 1\n" : "")
                + std::string(xp ? "#define OSLMI_CLOSURE_EXPORT 1\n" : "")
                + "#include \"oslmi_rt.h\"\n#include \"oslmi_groupargs.h\"\n"
                "using namespace oslmi;\nusing namespace oslmi::rt;\n"
@@ -1621,7 +1638,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
     std::ostringstream k;
     k << "// Emitted by the oslmi group emitter: persistent batch kernel over " << groups.size()
       << " groups.\n";
-    bool any_bb = false, any_mx = false;
+    bool any_bb = false, any_mx = false, any_aniso = false;
     std::vector<std::string> cases;
     for (size_t j = 0; j < groups.size(); ++j) {
         const int gi = ids[j];
@@ -1634,6 +1651,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
         }
         any_bb = any_bb || e.uses_bb;
         any_mx = any_mx || e.uses_mx;
+        any_aniso = any_aniso || e.uses_aniso;
         // a case keeps taking tiles while they belong to its group, so the
         // group's loop invariants live only inside its case
         std::ostringstream c;
@@ -1656,7 +1674,7 @@ bool emit_batch(const ShadingSys& ss, const std::vector<Group*>& groups,
         c << "        }\n        break;\n    }\n";
         cases.push_back(c.str());
     }
-    k << Emitter::prelude(any_mx);
+    k << Emitter::prelude(any_mx, false, any_aniso);
     k << "// next tile index for the whole workgroup (the barrier in front keeps\n"
          "// the previous index readable until every work-item has taken it)\n"
          "__device__ __forceinline__ int fetch_tile(int32_t* counter, int* s_tile) {\n"

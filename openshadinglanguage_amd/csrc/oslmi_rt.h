@@ -237,6 +237,11 @@ OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
     LobeList<1> c;
     Lobe l;
     l.id = kClosureMicrofacetGGX; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
+#ifdef OSLMI_FULL_MICROFACET
+    l.U       = V3(0.0f);
+    l.ay      = alpha;
+    l.refract = 0;
+#endif
 #ifdef OSLMI_CLOSURE_EXPORT
     {
         // MicrofacetParams: dist (0 ggx, 1 beckmann), N, U, xalpha, yalpha,
@@ -256,6 +261,11 @@ OSLMI_DEV LobeList<1> closure_microfacet_beckmann(V3 N, float alpha, float eta)
     LobeList<1> c;
     Lobe l;
     l.id = kClosureMicrofacetBeckmann; l.weight = V3(1.0f); l.N = N; l.alpha = alpha; l.eta = eta;
+#ifdef OSLMI_FULL_MICROFACET
+    l.U       = V3(0.0f);
+    l.ay      = alpha;
+    l.refract = 0;
+#endif
 #ifdef OSLMI_CLOSURE_EXPORT
     {
         // MicrofacetParams: dist (0 ggx, 1 beckmann), N, U, xalpha, yalpha,
@@ -270,6 +280,34 @@ OSLMI_DEV LobeList<1> closure_microfacet_beckmann(V3 N, float alpha, float eta)
 #endif
     return c;
 }
+#ifdef OSLMI_FULL_MICROFACET
+// microfacet(dist, N, U, xalpha, yalpha, eta, refract) in general: a
+// tangent, two roughnesses, refraction (1) or both reflection and
+// refraction (2): testrender's Microfacet<Dist, refract> in the frame
+// from_normal_and_tangent(N, U) (src/testrender/shading.cpp:570-800)
+OSLMI_DEV LobeList<1> closure_microfacet_full(int id, V3 N, V3 U, float xalpha, float yalpha,
+                                              float eta, int refract)
+{
+    LobeList<1> c;
+    Lobe l;
+    l.id = id; l.weight = V3(1.0f); l.N = N; l.alpha = xalpha; l.eta = eta;
+    l.U       = U;
+    l.ay      = yalpha;
+    l.refract = refract;
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        const float v[11] = { id == kClosureMicrofacetBeckmann ? 1.0f : 0.0f, N.x, N.y, N.z, U.x,
+                              U.y, U.z, xalpha, yalpha, eta, (float)refract };
+        set_xp(l, v, 11);
+    }
+#endif
+    c.add(l);
+#ifdef OSLMI_MX_CLOSURES
+    mark_component(c);
+#endif
+    return c;
+}
+#endif
 // color * closure (a MUL node): scales every lobe weight. A zero weight
 // gives the null closure (osl_mul_closure_color, opclosure.cpp:40-60); the
 // layer walks see the difference, lobe lists without MaterialX closures

@@ -119,13 +119,14 @@ BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
         const float pdf = std::max<float>(dot(l.N, wi), 0.0f) * float(M_1_PI);
         return { V3(1.0f), pdf };
     }
-    Frame tf        = frame_from_normal_tangent(l.N, V3(0.0f));
+    Frame tf        = frame_from_normal_tangent(l.N, l.aniso ? l.U : V3(0.0f));
     const V3 wo_l   = tolocal(tf, wo);
     const V3 wi_l   = tolocal(tf, wi);
-    const float xa  = l.alpha, ya = l.alpha;
-    if (wo_l.z > 0 && wi_l.z > 0) {
+    const float xa  = l.alpha, ya = l.aniso ? l.ay : l.alpha;
+    const bool bk   = l.kind == kLobeBeckmannRefl;
+    // Microfacet<Dist, Refract>::eval (src/testrender/shading.cpp:589-646)
+    if ((l.refract == 0 || l.refract == 2) && wo_l.z > 0 && wi_l.z > 0) {
         const V3 m           = normalize(wi_l + wo_l);
-        const bool bk        = l.kind == kLobeBeckmannRefl;
         const float D        = bk ? beckmann_D(m, xa, ya) : ggx_D(m, xa, ya);
         const float Lambda_o = bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya);
         const float Lambda_i = bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya);
@@ -134,9 +135,65 @@ BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
         const float Fr       = fresnel_dielectric(dot(m, wo_l), l.eta);
         float pdf            = (G1 * D * 0.25f) / wo_l.z;
         float out            = G2 / G1;
+        if (l.refract == 2) {
+            pdf *= Fr;
+            return { V3(out), pdf };
+        }
         return { V3(out * Fr), pdf };
     }
+    if ((l.refract == 1 || l.refract == 2) && wi_l.z < 0 && wo_l.z > 0.0f) {
+        // half-vector of the refraction (Walter et al. 2007, eq. 16)
+        V3 ht = -(wi_l * l.eta + wo_l);
+        if (l.eta < 1.0f)
+            ht = -ht;
+        const V3 Ht       = normalize(ht);
+        const float cosHO = dot(Ht, wo_l);
+        const float Ft    = 1.0f - fresnel_dielectric(cosHO, l.eta);
+        if (Ft > 0) {
+            const float cosHI = dot(Ht, wi_l);
+            if (Ht.z <= 0.0f)
+                return { V3(0.0f), 0.0f };
+            const float Dt       = bk ? beckmann_D(Ht, xa, ya) : ggx_D(Ht, xa, ya);
+            const float Lambda_o = bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya);
+            const float Lambda_i = bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya);
+            const float G2       = 1 / (Lambda_o + Lambda_i + 1);
+            const float G1       = 1 / (Lambda_o + 1);
+            const float invHt2   = 1 / dot(ht, ht);
+            float pdf = (std::fabs(cosHI * cosHO) * (l.eta * l.eta) * (G1 * Dt) * invHt2) / wo_l.z;
+            float out = G2 / G1;
+            if (l.refract == 2) {
+                pdf *= Ft;
+                return { V3(out), pdf };
+            }
+            return { V3(out * Ft), pdf };
+        }
+    }
     return { V3(0.0f), 0.0f };
+}
+
+// fresnel_refraction (src/testrender/optics.h:33-62)
+float fresnel_refraction(V3 I, V3 N, float eta, V3& T)
+{
+    float cosi = -dot(I, N);
+    V3 Nn;
+    float neta;
+    if (cosi > 0) {
+        neta = 1 / eta;
+        Nn   = N;
+    } else {
+        cosi = -cosi;
+        neta = eta;
+        Nn   = -N;
+    }
+    float arg = 1.0f - (neta * neta * (1.0f - cosi * cosi));
+    if (arg >= 0) {
+        float dnp = sqrtf(arg);
+        float nK  = (neta * cosi) - dnp;
+        T         = I * neta + Nn * nK;
+        return 1 - fresnel_dielectric(cosi, eta);
+    }
+    T = V3(0.0f);
+    return 0;
 }
 
 // TangentFrame::get: x u + y v + z w
@@ -184,7 +241,37 @@ void ggx_sample_slope(float cos_theta, float u, float v, float* sx, float* sy)
     *sy = copysignf(1.0f, Ru) * z * sqrtf(1.0f + *sx * *sx);
 }
 
-V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
+// Beckmann visible-slope sampling (BeckmannDist::sampleSlope,
+// src/testrender/shading.cpp:516-566: Jakob's improved routine, a coarse
+// root, one Newton step and a second when the first leaves a residual),
+// with the oracle's erf / erf^-1 / exp standing in for OpenImageIO's fast_
+// versions
+void beckmann_sample_slope(float cos_theta, float u, float v, float* sx, float* sy)
+{
+    const float sqrt_pi_inv = 1 / sqrtf(float(M_PI));
+    const float ct          = cos_theta < 1e-6f ? float(1e-6f) : cos_theta;
+    const float tan_i       = sqrtf(1 - ct * ct) / ct;
+    const float cot_i       = 1 / tan_i;
+    const float c           = ofast_erff(cot_i);
+    const float K           = tan_i * sqrt_pi_inv;
+    const float y_approx    = u * (1.0f + c + K * (1 - c * c));
+    const float y_exact     = u * (1.0f + c + K * ofast_expf(-cot_i * cot_i));
+    float b = K > 0 ? (0.5f - sqrtf(K * (K - y_approx + 1.0f) + 0.25f)) / K : y_approx - 1.0f;
+    float inv_erf = ofast_ierff(b);
+    float value   = 1.0f + b + K * ofast_expf(-inv_erf * inv_erf) - y_exact;
+    if (fabsf(value) > 1e-6f) {
+        b -= value / (1 - inv_erf * tan_i);
+        inv_erf = ofast_ierff(b);
+        value   = 1.0f + b + K * ofast_expf(-inv_erf * inv_erf) - y_exact;
+        b -= value / (1 - inv_erf * tan_i);
+        *sx = ofast_ierff(b);
+    } else {
+        *sx = inv_erf;
+    }
+    *sy = ofast_ierff(2.0f * v - 1.0f);
+}
+
+V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v, bool beckmann)
 {
     V3 swo(wo.x * xa, wo.y * ya, wo.z);
     swo                   = normalize(swo);
@@ -196,7 +283,10 @@ V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
         sin_phi         = swo.y * inv;
     }
     float slx, sly;
-    ggx_sample_slope(cos_theta, u, v, &slx, &sly);
+    if (beckmann)
+        beckmann_sample_slope(cos_theta, u, v, &slx, &sly);
+    else
+        ggx_sample_slope(cos_theta, u, v, &slx, &sly);
     float tx         = cos_phi * slx - sin_phi * sly;
     float ty         = sin_phi * slx + cos_phi * sly;
     tx               = tx * xa;
@@ -207,7 +297,7 @@ V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
               1.0f / mlen);
 }
 
-BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
+BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, float rz)
 {
     BsdfDirSample r = { V3(0.0f), V3(0.0f), 0.0f };
     if (l.kind == kLobeMx) {
@@ -222,26 +312,55 @@ BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry)
         r.pdf    = cos_theta * float(M_1_PI);
         return r;
     }
-    if (l.kind == kLobeBeckmannRefl)
-        return r;  // Beckmann lobes are evaluated, not sampled
-    const Frame tf    = frame_from_normal_tangent(l.N, V3(0.0f));
+    const Frame tf    = frame_from_normal_tangent(l.N, l.aniso ? l.U : V3(0.0f));
     const V3 wo_l     = tolocal(tf, wo);
     const float cosNO = wo_l.z;
     if (!(cosNO > 0))
         return r;
-    const float xa = l.alpha, ya = l.alpha;
-    const V3 m           = ggx_sample_micronormal(wo_l, xa, ya, rx, ry);
+    const float xa = l.alpha, ya = l.aniso ? l.ay : l.alpha;
+    const bool bk        = l.kind == kLobeBeckmannRefl;
+    const V3 m           = ggx_sample_micronormal(wo_l, xa, ya, rx, ry, bk);
     const float cosMO    = dot(m, wo_l);
     const float F        = fresnel_dielectric(cosMO, l.eta);
-    const V3 wi_l        = m * (2.0f * cosMO) - wo_l;
-    const float D        = ggx_D(m, xa, ya);
-    const float Lambda_o = ggx_lambda(wo_l, xa, ya);
-    const float Lambda_i = ggx_lambda(wi_l, xa, ya);
+    // Microfacet<GGX, Refract>::sample (src/testrender/shading.cpp:650-712)
+    if (l.refract == 0 || (l.refract == 2 && rz < F)) {
+        const V3 wi_l        = m * (2.0f * cosMO) - wo_l;
+        const float D        = (bk ? beckmann_D(m, xa, ya) : ggx_D(m, xa, ya));
+        const float Lambda_o = (bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya));
+        const float Lambda_i = (bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya));
+        const float G2       = 1 / (Lambda_o + Lambda_i + 1);
+        const float G1       = 1 / (Lambda_o + 1);
+        r.wi                 = frame_get(tf, wi_l.x, wi_l.y, wi_l.z);
+        r.pdf                = (G1 * D * 0.25f) / cosNO;
+        if (l.refract == 2) {
+            r.pdf *= F;
+            r.weight = V3(G2 / G1);
+        } else {
+            r.weight = V3(F * (G2 / G1));
+        }
+        return r;
+    }
+    const V3 M = frame_get(tf, m.x, m.y, m.z);
+    V3 wi;
+    const float Ft       = fresnel_refraction(-wo, M, l.eta, wi);
+    const V3 wi_l        = tolocal(tf, wi);
+    const float cosHO    = dot(m, wo_l);
+    const float cosHI    = dot(m, wi_l);
+    const float D        = (bk ? beckmann_D(m, xa, ya) : ggx_D(m, xa, ya));
+    const float Lambda_o = (bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya));
+    const float Lambda_i = (bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya));
     const float G2       = 1 / (Lambda_o + Lambda_i + 1);
     const float G1       = 1 / (Lambda_o + 1);
-    r.wi                 = frame_get(tf, wi_l.x, wi_l.y, wi_l.z);
-    r.pdf                = (G1 * D * 0.25f) / cosNO;
-    r.weight             = V3(F * (G2 / G1));
+    const V3 ht          = -(wi_l * l.eta + wo_l);
+    const float invHt2   = 1.0f / dot(ht, ht);
+    r.wi  = wi;
+    r.pdf = (std::fabs(cosHI * cosHO) * (l.eta * l.eta) * (G1 * D) * invHt2) / std::fabs(wo_l.z);
+    if (l.refract == 2) {
+        r.pdf *= Ft;
+        r.weight = V3(G2 / G1);
+    } else {
+        r.weight = V3(Ft * (G2 / G1));
+    }
     return r;
 }
 
@@ -251,8 +370,11 @@ V3 lobe_albedo(const Lobe& l, V3 wo)
         return mtx::mx_albedo(l.mx);
     if (l.kind == kLobeDiffuse)
         return V3(1.0f);
+    // Microfacet::get_albedo (shading.cpp:580-588)
+    if (l.refract == 2)
+        return V3(1.0f);
     float fr = fresnel_dielectric(dot(l.N, wo), l.eta);
-    return V3(fr);
+    return V3(l.refract ? 1 - fr : fr);
 }
 
 void update_eval(V3* w, float* pdf, V3 ow, float opdf, float b)
@@ -313,7 +435,7 @@ BsdfDirSample Composite::sample(V3 wo, float rx, float ry, float rz) const
         if (rx < (pdfs[i] + accum)) {
             rx = (rx - accum) / pdfs[i];
             rx = std::min<float>(rx, 0.99999994f);
-            BsdfDirSample s = lobe_sample(lobes[i], wo, rx, ry);
+            BsdfDirSample s = lobe_sample(lobes[i], wo, rx, ry, rz);
             const V3 wsc    = lobes[i].weight * (1 / pdfs[i]);
             s.weight        = s.weight * wsc;
             s.pdf           = s.pdf * pdfs[i];

@@ -12,10 +12,17 @@ struct Lobe {
     int kind;
     V3 weight;
     V3 N;
-    float alpha;  // microfacet xalpha = yalpha
+    float alpha;  // microfacet xalpha
     float eta;
     mtx::MxLobe mx;    // kLobeMx: a MaterialX lobe built at the shading point's wo
     mtx::MxParams mxp;  // kLobeMx: its closure parameters
+    // microfacet(dist, N, U, xalpha, yalpha, ...): with aniso set, the lobe's
+    // frame is from_normal_and_tangent(N, U) and yalpha is ay; otherwise
+    // yalpha = alpha about N
+    bool aniso  = false;
+    V3 U        = V3(0.0f);
+    float ay    = 0.0f;
+    int refract = 0;  // Microfacet<Dist, refract>: 0 reflect, 1 refract, 2 both
 };
 
 struct BsdfSample {

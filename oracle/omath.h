@@ -47,6 +47,26 @@ inline float ofast_safe_powf(float x, float y)
     return std::pow(x, y);
 }
 inline float ofast_cbrtf(float x) { return std::cbrt(x); }
+inline float ofast_logf(float x) { return std::log(x); }
+inline float ofast_erff(float x) { return std::erf(x); }
+// the inverse of erf by Newton steps on std::erf from a bisection start
+inline float ofast_ierff(float x)
+{
+    if (!(x > -1.0f && x < 1.0f))
+        return x * HUGE_VAL;
+    float lo = -6.0f, hi = 6.0f;
+    for (int i = 0; i < 60; ++i) {
+        const float mid = 0.5f * (lo + hi);
+        if (std::erf(mid) < x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    float y = 0.5f * (lo + hi);
+    for (int i = 0; i < 3; ++i)
+        y -= (std::erf(y) - x) / (1.1283791670955126 * std::exp(-y * y));
+    return y;
+}
 inline float osafe_powf(float x, float y)
 {
     if (y == 0.0f)
@@ -266,6 +286,86 @@ inline float ofast_cbrtf(float x)
         y        = y - (y2 * y - ax) / (3.0f * y2);
     }
     return x < 0.0f ? -y : y;
+}
+
+// ln x for finite x > 0: the reduction of ofast_log2f, e ln2 added last.
+inline float ofast_logf(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    int e = 0;
+    if (u < 0x00800000u) {
+        x = x * 8388608.0f;
+        std::memcpy(&u, &x, 4);
+        e = -23;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = bits_f((u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421353816986083984375f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    float s  = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p  = 1.0f / 9.0f;
+    p        = fmaf(p, s2, 1.0f / 7.0f);
+    p        = fmaf(p, s2, 1.0f / 5.0f);
+    p        = fmaf(p, s2, 1.0f / 3.0f);
+    p        = fmaf(p, s2, 1.0f);
+    float ln = 2.0f * s * p;
+    return fmaf((float)e, 0.693147182464599609375f, ln);
+}
+
+// erf by Abramowitz and Stegun 7.1.28: 1 - (1 + a1|x| + ... + a6|x|^6)^-16
+// with the sign of x (absolute error <= 3e-7 in exact arithmetic).
+inline float ofast_erff(float x)
+{
+    const float ax = std::fabs(x);
+    float p        = 0.0000430638f;
+    p              = fmaf(p, ax, 0.0002765672f);
+    p              = fmaf(p, ax, 0.0001520143f);
+    p              = fmaf(p, ax, 0.0092705272f);
+    p              = fmaf(p, ax, 0.0422820123f);
+    p              = fmaf(p, ax, 0.0705230784f);
+    p              = fmaf(p, ax, 1.0f);
+    const float p2 = p * p, p4 = p2 * p2, p8 = p4 * p4, p16 = p8 * p8;
+    return std::copysign(1.0f - 1.0f / p16, x);
+}
+
+// erf^-1 by M. Giles, "Approximating the erfinv function" (GPU Computing
+// Gems, 2011), single-precision branch pair in w = -ln((1-x)(1+x)); x = +-1
+// (or beyond) gives an infinity of x's sign.
+inline float ofast_ierff(float x)
+{
+    const float q = (1.0f - x) * (1.0f + x);
+    if (!(q > 0.0f))
+        return x * bits_f(0x7f800000u);
+    float w = -ofast_logf(q);
+    float p;
+    if (w < 5.0f) {
+        w = w - 2.5f;
+        p = 2.81022636e-08f;
+        p = fmaf(p, w, 3.43273939e-07f);
+        p = fmaf(p, w, -3.5233877e-06f);
+        p = fmaf(p, w, -4.39150654e-06f);
+        p = fmaf(p, w, 0.00021858087f);
+        p = fmaf(p, w, -0.00125372503f);
+        p = fmaf(p, w, -0.00417768164f);
+        p = fmaf(p, w, 0.246640727f);
+        p = fmaf(p, w, 1.50140941f);
+    } else {
+        w = std::sqrt(w) - 3.0f;
+        p = -0.000200214257f;
+        p = fmaf(p, w, 0.000100950558f);
+        p = fmaf(p, w, 0.00134934322f);
+        p = fmaf(p, w, -0.00367342844f);
+        p = fmaf(p, w, 0.00573950773f);
+        p = fmaf(p, w, -0.0076224613f);
+        p = fmaf(p, w, 0.00943887047f);
+        p = fmaf(p, w, 1.00167406f);
+        p = fmaf(p, w, 2.83297682f);
+    }
+    return p * x;
 }
 #endif  // OREF_F64
 

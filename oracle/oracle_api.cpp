@@ -289,28 +289,44 @@ void oref_math(int fn, const float* in, float* out, int64_t n)
         case 0: out[i] = ofast_expf(in[i]); break;
         case 1: ofast_sincosf(in[i], &s, &c); out[i] = s; break;
         case 2: ofast_sincosf(in[i], &s, &c); out[i] = c; break;
+        case 4: out[i] = ofast_logf(in[i]); break;
+        case 5: out[i] = ofast_erff(in[i]); break;
+        case 6: out[i] = ofast_ierff(in[i]); break;
         default: out[i] = ofast_cbrtf(in[i]); break;
         }
     }
 }
 
+// A lobe from a row of 9 floats (kind, weight rgb, N xyz, alpha, eta), 13
+// (the same, then the tangent U xyz and yalpha of an anisotropic microfacet
+// lobe) or 14 (then the refract mode 0 / 1 / 2).
+static Lobe lobe_row(const float* r, int cols)
+{
+    Lobe l;
+    l.kind   = (int)r[0];
+    l.weight = V3(r[1], r[2], r[3]);
+    l.N      = V3(r[4], r[5], r[6]);
+    l.alpha  = r[7];
+    l.eta    = r[8];
+    if (cols >= 13) {
+        l.aniso = true;
+        l.U     = V3(r[9], r[10], r[11]);
+        l.ay    = r[12];
+    }
+    if (cols >= 14)
+        l.refract = (int)r[13];
+    return l;
+}
+
 // CompositeBSDF prepare + eval over one lobe list for n direction pairs.
-// lobes: nlobes rows of 9 floats (kind, weight rgb, N xyz, alpha, eta);
-// wo, wi: 3 planes of n; out: 4 planes (weight rgb, pdf).
+// lobes: nlobes rows of cols (9 or 13) floats, see lobe_row; wo, wi: 3
+// planes of n; out: 4 planes (weight rgb, pdf).
 void oref_composite_eval(int nlobes, const float* lobes, const float* wo, const float* wi,
-                         float* out, int64_t n)
+                         float* out, int64_t n, int cols)
 {
     Composite c;
-    for (int k = 0; k < nlobes; ++k) {
-        const float* r = lobes + 9 * k;
-        Lobe l;
-        l.kind   = (int)r[0];
-        l.weight = V3(r[1], r[2], r[3]);
-        l.N      = V3(r[4], r[5], r[6]);
-        l.alpha  = r[7];
-        l.eta    = r[8];
-        c.add(l);
-    }
+    for (int k = 0; k < nlobes; ++k)
+        c.add(lobe_row(lobes + cols * k, cols));
     for (int64_t i = 0; i < n; ++i) {
         Composite ci = c;
         const V3 o(wo[i], wo[n + i], wo[2 * n + i]);
@@ -327,19 +343,11 @@ void oref_composite_eval(int nlobes, const float* lobes, const float* wo, const 
 // CompositeBSDF prepare + sample over one lobe list: wo 3 planes, rnd 3
 // planes (rx, ry, rz); out 7 planes (wi xyz, weight rgb, pdf).
 void oref_composite_sample(int nlobes, const float* lobes, const float* wo, const float* rnd,
-                           float* out, int64_t n)
+                           float* out, int64_t n, int cols)
 {
     Composite c;
-    for (int k = 0; k < nlobes; ++k) {
-        const float* r = lobes + 9 * k;
-        Lobe l;
-        l.kind   = (int)r[0];
-        l.weight = V3(r[1], r[2], r[3]);
-        l.N      = V3(r[4], r[5], r[6]);
-        l.alpha  = r[7];
-        l.eta    = r[8];
-        c.add(l);
-    }
+    for (int k = 0; k < nlobes; ++k)
+        c.add(lobe_row(lobes + cols * k, cols));
     for (int64_t i = 0; i < n; ++i) {
         Composite ci = c;
         const V3 o(wo[i], wo[n + i], wo[2 * n + i]);

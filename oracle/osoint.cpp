@@ -853,10 +853,15 @@ struct Exec {
                     l.alpha = 0.0f;
                     l.eta   = 0.0f;
                 } else {
+                    // microfacet(dist, N, U, xalpha, yalpha, eta, refract)
                     l.kind  = Sv(L, a(2)) == "beckmann" ? kLobeBeckmannRefl : kLobeGGXRefl;
                     l.N     = T(L, a(3));
                     l.alpha = F(L, a(5));
                     l.eta   = F(L, a(7));
+                    l.U     = T(L, a(4));
+                    l.ay    = F(L, a(6));
+                    l.aniso = true;  // the frame of (N, U) is from_normal(N) when U = 0
+                    l.refract = I(L, a(8));
                 }
             }
             cn.push_back(nd);
@@ -1273,11 +1278,11 @@ void oref_group_closures(void* h, const float* P, const float* N, const float* d
                 v[0] = 8;  // MICROFACET_ID: dist, N, U, xalpha, yalpha, eta, refract
                 *p++ = l.kind == kLobeBeckmannRefl ? 1.0f : 0.0f;
                 put3(l.N);
-                put3(V3(0.0f));
+                put3(l.aniso ? l.U : V3(0.0f));
                 *p++ = l.alpha;
-                *p++ = l.alpha;
+                *p++ = l.aniso ? l.ay : l.alpha;
                 *p++ = l.eta;
-                *p++ = 0.0f;
+                *p++ = (float)l.refract;
             } else {
                 const mtx::MxParams& m = l.mxp;
                 if (m.kind == 2) {

@@ -64,6 +64,10 @@ class _W:
         return self.closure("microfacet", self.o.const("string", "ggx"), "N", self.zero_v(), alpha,
                             alpha, self.o.const("float", 1.5), self.o.const("int", 0))
 
+    def ggx_aniso(self, U, xalpha, yalpha, dist="ggx", refract=0, eta=1.5):
+        return self.closure("microfacet", self.o.const("string", dist), "N", U, xalpha, yalpha,
+                            self.o.const("float", eta), self.o.const("int", refract))
+
     def layer(self, top, base):
         return self.closure("layer", top, base)
 
@@ -150,6 +154,26 @@ def programs():
     x, y, z = w.f
     c = w.dielectric(w.lin(y, 0.05, 0.9), w.lin(x, 0.05, 0.9), w.lin(z, 1.3, 0.5))
     out.append(("mx_reuse", w.done(w.add(w.layer(c, c), w.mul(w.fconst(0.5), c)))))
+
+    # a dielectric coat over an anisotropic GGX base (tangent dPdu, the two
+    # roughnesses from P), plus an isotropic GGX lobe in the same list
+    w = _W("mx_coat_aniso")
+    x, y, z = w.f
+    base = w.ggx_aniso("dPdu", w.lin(x, 0.05, 0.5), w.lin(y, 0.05, 0.5))
+    coat = w.dielectric(w.lin(z, 0.05, 0.5), w.lin(z, 0.05, 0.5), w.fconst(1.5))
+    out.append(("mx_coat_aniso", w.done(w.add(w.layer(coat, base),
+                                              w.mul(w.fconst(0.3), w.ggx(w.lin(z, 0.1, 0.4)))))))
+
+    # glass: a sheen over a rough anisotropic GGX that reflects and refracts
+    # (refract 2), plus a refract-only isotropic lobe from the inside
+    # (eta < 1)
+    w = _W("mx_glass")
+    x, y, z = w.f
+    glass = w.ggx_aniso("dPdv", w.lin(x, 0.05, 0.4), w.lin(y, 0.05, 0.4), refract=2, eta=1.45)
+    inner = w.ggx_aniso(w.zero_v(), w.lin(z, 0.1, 0.3), w.lin(z, 0.1, 0.3), refract=1,
+                        eta=1.0 / 1.33)
+    out.append(("mx_glass", w.done(w.add(w.layer(w.sheen(w.lin(z, 0.1, 0.5)), glass),
+                                         w.mul(w.fconst(0.5), inner)))))
     return out
 
 

@@ -198,12 +198,14 @@ def shade_c2(P, N, wo, wi, dPdx, dPdy, nthreads=1, debug=False):
 
 
 def omath(fn, x):
-    """The oracle's fixed transcendentals: 'exp', 'sin', 'cos', 'cbrt'."""
+    """The oracle's fixed transcendentals: 'exp', 'sin', 'cos', 'cbrt', 'log',
+    'erf', 'ierf'."""
     L = lib()
     L.oref_math.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
     x = np.ascontiguousarray(x, dtype=np.float32)
     out = np.zeros_like(x)
-    L.oref_math(["exp", "sin", "cos", "cbrt"].index(fn), _p(x), _p(out), x.size)
+    L.oref_math(["exp", "sin", "cos", "cbrt", "log", "erf", "ierf"].index(fn), _p(x), _p(out),
+                x.size)
     return out
 
 
@@ -349,16 +351,19 @@ class Group:
 
 def composite_eval(lobes, wo, wi):
     """CompositeBSDF prepare + eval of one lobe list at n direction pairs.
-    lobes: rows (kind, w_r, w_g, w_b, Nx, Ny, Nz, alpha, eta); kind 0
-    diffuse, 1 GGX reflection. Returns 4 planes (weight rgb, pdf)."""
+    lobes: rows (kind, w_r, w_g, w_b, Nx, Ny, Nz, alpha, eta), optionally
+    followed by (Ux, Uy, Uz, yalpha) for an anisotropic microfacet lobe;
+    kind 0 diffuse, 1 GGX reflection, 2 Beckmann. Returns 4 planes (weight
+    rgb, pdf)."""
     L = lib()
     L.oref_composite_eval.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
-    lo = np.ascontiguousarray(lobes, np.float32)
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_int]
+    lo = np.ascontiguousarray(np.atleast_2d(lobes), np.float32)
     a = [np.ascontiguousarray(x, np.float32) for x in (wo, wi)]
     n = a[0].shape[1]
     out = np.zeros((4, n), np.float32)
-    L.oref_composite_eval(lo.shape[0], _p(lo), _p(a[0]), _p(a[1]), _p(out), n)
+    L.oref_composite_eval(lo.shape[0], _p(lo), _p(a[0]), _p(a[1]), _p(out), n, lo.shape[1])
     return out
 
 
@@ -483,12 +488,13 @@ def composite_sample(lobes, wo, rnd):
     rgb, pdf)."""
     L = lib()
     L.oref_composite_sample.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
-    lo = np.ascontiguousarray(lobes, np.float32)
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.c_int]
+    lo = np.ascontiguousarray(np.atleast_2d(lobes), np.float32)
     a = [np.ascontiguousarray(x, np.float32) for x in (wo, rnd)]
     n = a[0].shape[1]
     out = np.zeros((7, n), np.float32)
-    L.oref_composite_sample(lo.shape[0], _p(lo), _p(a[0]), _p(a[1]), _p(out), n)
+    L.oref_composite_sample(lo.shape[0], _p(lo), _p(a[0]), _p(a[1]), _p(out), n, lo.shape[1])
     return out
 
 

@@ -145,16 +145,21 @@ def _microfacet_shader(xa, ya, u):
     return o
 
 
-def test_anisotropic_microfacet_is_rejected():
+def test_anisotropic_microfacet_builds_the_tangent_frame_lobe():
+    """Equal roughnesses about N keep the short isotropic lobe; two
+    roughnesses or a tangent build the anisotropic lobe (the kernel then
+    defines OSLMI_FULL_MICROFACET), which isotropic kernels never carry."""
     ss = S.ShadingSystem()
     ok = _microfacet_shader(0.3, 0.3, (0, 0, 0))
     g = S.build_generated_group(ss, ([(ok.name, "l0", ok.text())], []), "mf_ok")
-    assert "closure_microfacet_ggx" in ss.emit(g)
-    for xa, ya, u in ((0.3, 0.2, (0, 0, 0)), (0.3, 0.3, (1, 0, 0))):
+    src = ss.emit(g)
+    assert "closure_microfacet_ggx" in src and "OSLMI_FULL_MICROFACET" not in src
+    for k, (xa, ya, u) in enumerate(((0.3, 0.2, (0, 0, 0)), (0.3, 0.3, (1, 0, 0)))):
         o = _microfacet_shader(xa, ya, u)
-        g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), "mf_bad")
-        with pytest.raises(OSLMIError):
-            ss.emit(g)
+        g = S.build_generated_group(ss, ([(o.name, "l0", o.text())], []), "mf_aniso%d" % k)
+        src = ss.emit(g)
+        assert "closure_microfacet_full(kClosureMicrofacetGGX" in src
+        assert "#define OSLMI_FULL_MICROFACET 1" in src
 
 
 def _beckmann_shader():
