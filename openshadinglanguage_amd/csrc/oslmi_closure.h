@@ -41,14 +41,15 @@ namespace oslmi {
 // Lobe kinds (the library's own numbering).
 enum ClosureId {
     kClosureDiffuse = 1, kClosureMicrofacetGGX = 2, kClosureMicrofacetBeckmann = 3,
-    kClosureMxDielectric = 4, kClosureMxSchlick = 5, kClosureMxSheen = 6
+    kClosureTranslucent = 4, kClosurePhong = 5,
+    kClosureMxDielectric = 6, kClosureMxSchlick = 7, kClosureMxSheen = 8
 };
 
 struct Lobe {
     int id;
     V3 weight;
-    V3 N;
-    float alpha;  // microfacet xalpha (and yalpha unless anisotropic)
+    V3 N;         // translucent: the closure's N negated (testrender's Diffuse<1>)
+    float alpha;  // microfacet xalpha (and yalpha unless anisotropic); phong exponent
     float eta;
 #ifdef OSLMI_FULL_MICROFACET
     // general microfacet lobes: the tangent U of the lobe's frame
@@ -281,6 +282,27 @@ OSLMI_DEV float beckmann_D(V3 Hr, float xa, float ya)
     return 0;
 }
 
+// Phong::eval (src/testrender/shading.cpp:323-340): a cosine-power lobe
+// about the mirror direction of wo; exponent in alpha
+OSLMI_DEV BsdfEval phong_eval(const Lobe& l, V3 wo, V3 wi)
+{
+    BsdfEval r;
+    r.weight          = V3(0.0f);
+    r.pdf             = 0.0f;
+    const float cosNI = dot(l.N, wi);
+    const float cosNO = dot(l.N, wo);
+    if (cosNI > 0 && cosNO > 0) {
+        const V3 R        = l.N * (2 * cosNO) - wo;
+        const float cosRI = dot(R, wi);
+        if (cosRI > 0) {
+            const float e = l.alpha;
+            r.pdf         = (e + 1) * 0.15915493667125701904f * fsafe_pow(cosRI, e);  // float(M_1_PI / 2)
+            r.weight      = V3(cosNI * (e + 2) / (e + 1));
+        }
+    }
+    return r;
+}
+
 OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
 {
     BsdfEval r;
@@ -294,11 +316,13 @@ OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
         return r;
     }
 #endif
-    if (l.id == kClosureDiffuse) {
+    if (l.id == kClosureDiffuse || l.id == kClosureTranslucent) {
         r.weight = V3(1.0f);
         r.pdf    = fmaxf(dot(l.N, wi), 0.0f) * 0.31830987334251403809f;  // float(M_1_PI)
         return r;
     }
+    if (l.id == kClosurePhong)
+        return phong_eval(l, wo, wi);
     Frame tf       = lobe_frame(l);
     V3 wo_l        = tolocal(tf, wo);
     V3 wi_l        = tolocal(tf, wi);
@@ -367,8 +391,8 @@ OSLMI_DEV V3 lobe_albedo(const Lobe& l, V3 wo)
     if (l.id >= kClosureMxDielectric)
         return mtx::mx_albedo(l.mx);
 #endif
-    if (l.id == kClosureDiffuse)
-        return V3(1.0f);
+    if (l.id == kClosureDiffuse || l.id == kClosureTranslucent || l.id == kClosurePhong)
+        return V3(1.0f);  // BSDF::get_albedo (src/testrender/shading.h:287-290)
     // Microfacet::get_albedo (shading.cpp:580-588)
     const int rf = lobe_refract(l);
     if (rf == 2)
@@ -463,7 +487,37 @@ OSLMI_DEV void ggx_sample_slope(float cos_theta, float u, float v, float* sx, fl
     *sy = copysignf(1.0f, Ru) * z * sqrtf(1.0f + *sx * *sx);
 }
 
-OSLMI_DEV V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
+// Beckmann visible-slope sampling (BeckmannDist::sampleSlope,
+// src/testrender/shading.cpp:516-566: Jakob's improved routine, a coarse
+// root, one Newton step and a second when the first leaves a residual),
+// with the library's erf / erf^-1 / exp standing in for OpenImageIO's fast_
+// versions
+OSLMI_DEV void beckmann_sample_slope(float cos_theta, float u, float v, float* sx, float* sy)
+{
+    const float sqrt_pi_inv = 1 / sqrtf(3.14159274101257324219f);
+    const float ct          = cos_theta < 1e-6f ? 1e-6f : cos_theta;
+    const float tan_i       = sqrtf(1 - ct * ct) / ct;
+    const float cot_i       = 1 / tan_i;
+    const float c           = ferf(cot_i);
+    const float K           = tan_i * sqrt_pi_inv;
+    const float y_approx    = u * (1.0f + c + K * (1 - c * c));
+    const float y_exact     = u * (1.0f + c + K * fexp(-cot_i * cot_i));
+    float b = K > 0 ? (0.5f - sqrtf(K * (K - y_approx + 1.0f) + 0.25f)) / K : y_approx - 1.0f;
+    float inv_erf = fierf(b);
+    float value   = 1.0f + b + K * fexp(-inv_erf * inv_erf) - y_exact;
+    if (fabsf(value) > 1e-6f) {
+        b -= value / (1 - inv_erf * tan_i);
+        inv_erf = fierf(b);
+        value   = 1.0f + b + K * fexp(-inv_erf * inv_erf) - y_exact;
+        b -= value / (1 - inv_erf * tan_i);
+        *sx = fierf(b);
+    } else {
+        *sx = inv_erf;
+    }
+    *sy = fierf(2.0f * v - 1.0f);
+}
+
+OSLMI_DEV V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v, bool beckmann)
 {
     V3 swo(wo.x * xa, wo.y * ya, wo.z);
     const float sl = sqrtf(dot(swo, swo));
@@ -477,7 +531,10 @@ OSLMI_DEV V3 ggx_sample_micronormal(V3 wo, float xa, float ya, float u, float v)
         sin_phi         = swo.y * inv;
     }
     float slx, sly;
-    ggx_sample_slope(cos_theta, u, v, &slx, &sly);
+    if (beckmann)
+        beckmann_sample_slope(cos_theta, u, v, &slx, &sly);
+    else
+        ggx_sample_slope(cos_theta, u, v, &slx, &sly);
     float tx         = cos_phi * slx - sin_phi * sly;
     float ty         = sin_phi * slx + cos_phi * sly;
     tx               = tx * xa;
@@ -508,7 +565,27 @@ OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, fl
         return r;
     }
 #endif
-    if (l.id == kClosureDiffuse) {
+    if (l.id == kClosurePhong) {
+        // Phong::sample (shading.cpp:342-362): a cosine-power lobe about the
+        // mirror direction, then the eval there
+        const float cosNO = dot(l.N, wo);
+        if (cosNO > 0) {
+            const V3 R       = l.N * (2 * cosNO) - wo;
+            const float phi  = 2 * 3.14159274101257324219f * rx;
+            float sp, cp;
+            fsincos(phi, &sp, &cp);
+            const float cosTheta  = fsafe_pow(ry, 1 / (l.alpha + 1));
+            const float sinTheta2 = 1 - cosTheta * cosTheta;
+            const float sinTheta  = sinTheta2 > 0 ? sqrtf(sinTheta2) : 0;
+            const V3 wi = frame_get(frame_from_normal(R), cp * sinTheta, sp * sinTheta, cosTheta);
+            const BsdfEval e = phong_eval(l, wo, wi);
+            r.wi     = wi;
+            r.weight = e.weight;
+            r.pdf    = e.pdf;
+        }
+        return r;
+    }
+    if (l.id == kClosureDiffuse || l.id == kClosureTranslucent) {
         to_unit_disk(rx, ry);
         const float cos_theta = sqrtf(fmaxf(1 - rx * rx - ry * ry, 0.0f));
         r.wi     = frame_get(frame_from_normal(l.N), rx, ry, cos_theta);
@@ -523,16 +600,17 @@ OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, fl
         return r;
     const float xa    = l.alpha, ya = lobe_yalpha(l);
     const int rf      = lobe_refract(l);
-    const V3 m        = ggx_sample_micronormal(wo_l, xa, ya, rx, ry);
+    const bool bk     = l.id == kClosureMicrofacetBeckmann;
+    const V3 m        = ggx_sample_micronormal(wo_l, xa, ya, rx, ry, bk);
     const float cosMO = dot(m, wo_l);
     const float F     = fresnel_dielectric(cosMO, l.eta);
     if (rf == 0 || (rf == 2 && rz < F)) {
         // reflection; rz chooses it by the microfacet's Fresnel term when
         // the lobe both reflects and refracts
         const V3 wi_l  = m * (2.0f * cosMO) - wo_l;
-        const float D  = ggx_D(m, xa, ya);
-        const float Lo = ggx_lambda(wo_l, xa, ya);
-        const float Li = ggx_lambda(wi_l, xa, ya);
+        const float D  = (bk ? beckmann_D(m, xa, ya) : ggx_D(m, xa, ya));
+        const float Lo = (bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya));
+        const float Li = (bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya));
         const float G2 = 1 / (Lo + Li + 1);
         const float G1 = 1 / (Lo + 1);
         r.wi           = frame_get(tf, wi_l.x, wi_l.y, wi_l.z);
@@ -553,9 +631,9 @@ OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, fl
     const V3 wi_l     = tolocal(tf, wi);
     const float cosHO = dot(m, wo_l);
     const float cosHI = dot(m, wi_l);
-    const float D     = ggx_D(m, xa, ya);
-    const float Lo    = ggx_lambda(wo_l, xa, ya);
-    const float Li    = ggx_lambda(wi_l, xa, ya);
+    const float D     = (bk ? beckmann_D(m, xa, ya) : ggx_D(m, xa, ya));
+    const float Lo    = (bk ? beckmann_lambda(wo_l, xa, ya) : ggx_lambda(wo_l, xa, ya));
+    const float Li    = (bk ? beckmann_lambda(wi_l, xa, ya) : ggx_lambda(wi_l, xa, ya));
     const float G2    = 1 / (Lo + Li + 1);
     const float G1    = 1 / (Lo + 1);
     const V3 ht       = -(wi_l * l.eta + wo_l);
@@ -637,6 +715,8 @@ template<int MAXL> OSLMI_DEV BsdfEval composite_eval(const LobeList<MAXL>& L, V3
 OSLMI_DEV float export_id(int id)
 {
     return id == kClosureDiffuse ? 3.0f                        // DIFFUSE_ID
+           : id == kClosureTranslucent ? 5.0f                 // TRANSLUCENT_ID
+           : id == kClosurePhong ? 6.0f                       // PHONG_ID
            : id == kClosureMicrofacetGGX || id == kClosureMicrofacetBeckmann ? 8.0f  // MICROFACET_ID
            : id == kClosureMxDielectric ? 17.0f               // MX_DIELECTRIC_ID
            : id == kClosureMxSchlick ? 19.0f                  // MX_GENERALIZED_SCHLICK_ID

@@ -1236,15 +1236,19 @@ struct Emitter {
             strval(L, a(1), cname);
             if (cname == "diffuse") {
                 line(R + " = closure_diffuse(" + ex(L, a(2), K_V3) + ");");
+            } else if (cname == "translucent") {
+                line(R + " = closure_translucent(" + ex(L, a(2), K_V3) + ");");
+            } else if (cname == "phong") {
+                if (op.args.size() < 4) {
+                    fail("phong closure needs 2 arguments");
+                    return;
+                }
+                line(R + " = closure_phong(" + ex(L, a(2), K_V3) + ", " + ex(L, a(3), K_FLOAT) + ");");
             } else if (cname == "microfacet") {
                 std::string dist;
                 strval(L, a(2), dist);
                 if (dist != "ggx" && dist != "beckmann") {
                     fail("microfacet distribution '" + dist + "' is not supported");
-                    return;
-                }
-                if (dist == "beckmann" && ss.closure_sample) {
-                    fail("sampling a beckmann microfacet is not supported");
                     return;
                 }
                 // microfacet(dist, N, U, xalpha, yalpha, eta, refract)
@@ -1260,10 +1264,6 @@ struct Emitter {
                 const int refract = refr_const ? (int)refr[0] : -1;
                 if (refract < 0 || refract > 2) {
                     fail("microfacet refract must be a constant 0, 1 or 2");
-                    return;
-                }
-                if (refract != 0 && dist == "beckmann" && ss.closure_sample) {
-                    fail("sampling a beckmann microfacet is not supported");
                     return;
                 }
                 // isotropic lobes about N (yalpha the same symbol as xalpha,

@@ -115,6 +115,85 @@ OSLMI_DEV float flog2(float x)
     return __fmaf_rn(ln, 1.44269502162933349609375f, (float)e);
 }
 
+// ln x for finite x > 0: the reduction of flog2, e ln2 added last.
+OSLMI_DEV float flog(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    int e      = 0;
+    if (u < 0x00800000u) {
+        x = x * 8388608.0f;
+        u = __float_as_uint(x);
+        e = -23;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = __uint_as_float((u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421353816986083984375f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    float s  = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p  = 1.0f / 9.0f;
+    p        = __fmaf_rn(p, s2, 1.0f / 7.0f);
+    p        = __fmaf_rn(p, s2, 1.0f / 5.0f);
+    p        = __fmaf_rn(p, s2, 1.0f / 3.0f);
+    p        = __fmaf_rn(p, s2, 1.0f);
+    float ln = 2.0f * s * p;
+    return __fmaf_rn((float)e, 0.693147182464599609375f, ln);
+}
+
+// erf by Abramowitz and Stegun 7.1.28: 1 - (1 + a1|x| + ... + a6|x|^6)^-16
+// with the sign of x.
+OSLMI_DEV float ferf(float x)
+{
+    const float ax = fabsf(x);
+    float p        = 0.0000430638f;
+    p              = __fmaf_rn(p, ax, 0.0002765672f);
+    p              = __fmaf_rn(p, ax, 0.0001520143f);
+    p              = __fmaf_rn(p, ax, 0.0092705272f);
+    p              = __fmaf_rn(p, ax, 0.0422820123f);
+    p              = __fmaf_rn(p, ax, 0.0705230784f);
+    p              = __fmaf_rn(p, ax, 1.0f);
+    const float p2 = p * p, p4 = p2 * p2, p8 = p4 * p4, p16 = p8 * p8;
+    return copysignf(1.0f - 1.0f / p16, x);
+}
+
+// erf^-1 by M. Giles, "Approximating the erfinv function" (GPU Computing
+// Gems, 2011), single-precision branch pair in w = -ln((1-x)(1+x)); x = +-1
+// (or beyond) gives an infinity of x's sign.
+OSLMI_DEV float fierf(float x)
+{
+    const float q = (1.0f - x) * (1.0f + x);
+    if (!(q > 0.0f))
+        return x * __uint_as_float(0x7f800000u);
+    float w = -flog(q);
+    float p;
+    if (w < 5.0f) {
+        w = w - 2.5f;
+        p = 2.81022636e-08f;
+        p = __fmaf_rn(p, w, 3.43273939e-07f);
+        p = __fmaf_rn(p, w, -3.5233877e-06f);
+        p = __fmaf_rn(p, w, -4.39150654e-06f);
+        p = __fmaf_rn(p, w, 0.00021858087f);
+        p = __fmaf_rn(p, w, -0.00125372503f);
+        p = __fmaf_rn(p, w, -0.00417768164f);
+        p = __fmaf_rn(p, w, 0.246640727f);
+        p = __fmaf_rn(p, w, 1.50140941f);
+    } else {
+        w = sqrtf(w) - 3.0f;
+        p = -0.000200214257f;
+        p = __fmaf_rn(p, w, 0.000100950558f);
+        p = __fmaf_rn(p, w, 0.00134934322f);
+        p = __fmaf_rn(p, w, -0.00367342844f);
+        p = __fmaf_rn(p, w, 0.00573950773f);
+        p = __fmaf_rn(p, w, -0.0076224613f);
+        p = __fmaf_rn(p, w, 0.00943887047f);
+        p = __fmaf_rn(p, w, 1.00167406f);
+        p = __fmaf_rn(p, w, 2.83297682f);
+    }
+    return p * x;
+}
+
 // 2^t: t = k + f, k = rint(t), 2^f by the Taylor series of e^(f ln2) to
 // degree 7, 2^k in two exact scalings.
 OSLMI_DEV float fexp2(float t)

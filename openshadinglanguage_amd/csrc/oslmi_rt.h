@@ -232,6 +232,42 @@ OSLMI_DEV LobeList<1> closure_diffuse(V3 N)
 #endif
     return c;
 }
+// translucent(N): testrender's Diffuse<1>, a diffuse lobe about -N
+OSLMI_DEV LobeList<1> closure_translucent(V3 N)
+{
+    LobeList<1> c;
+    Lobe l;
+    l.id = kClosureTranslucent; l.weight = V3(1.0f); l.N = -N; l.alpha = 0.0f; l.eta = 0.0f;
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        const float v[3] = { N.x, N.y, N.z };  // DiffuseParams: N as given
+        set_xp(l, v, 3);
+    }
+#endif
+    c.add(l);
+#ifdef OSLMI_MX_CLOSURES
+    mark_component(c);
+#endif
+    return c;
+}
+// phong(N, exponent): testrender's Phong
+OSLMI_DEV LobeList<1> closure_phong(V3 N, float exponent)
+{
+    LobeList<1> c;
+    Lobe l;
+    l.id = kClosurePhong; l.weight = V3(1.0f); l.N = N; l.alpha = exponent; l.eta = 0.0f;
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        const float v[4] = { N.x, N.y, N.z, exponent };  // PhongParams: N, exponent
+        set_xp(l, v, 4);
+    }
+#endif
+    c.add(l);
+#ifdef OSLMI_MX_CLOSURES
+    mark_component(c);
+#endif
+    return c;
+}
 OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
 {
     LobeList<1> c;

@@ -115,9 +115,24 @@ BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
         const mtx::Sample s = mtx::mx_eval(l.mx, wo, wi);
         return { s.weight, s.pdf };
     }
-    if (l.kind == kLobeDiffuse) {
+    if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent) {
         const float pdf = std::max<float>(dot(l.N, wi), 0.0f) * float(M_1_PI);
         return { V3(1.0f), pdf };
+    }
+    if (l.kind == kLobePhong) {
+        // Phong::eval (src/testrender/shading.cpp:323-340)
+        const float cosNI = dot(l.N, wi);
+        const float cosNO = dot(l.N, wo);
+        if (cosNI > 0 && cosNO > 0) {
+            const V3 R        = l.N * (2 * cosNO) - wo;
+            const float cosRI = dot(R, wi);
+            if (cosRI > 0) {
+                const float e   = l.alpha;
+                const float pdf = (e + 1) * float(M_1_PI / 2) * ofast_safe_powf(cosRI, e);
+                return { V3(cosNI * (e + 2) / (e + 1)), pdf };
+            }
+        }
+        return { V3(0.0f), 0.0f };
     }
     Frame tf        = frame_from_normal_tangent(l.N, l.aniso ? l.U : V3(0.0f));
     const V3 wo_l   = tolocal(tf, wo);
@@ -304,7 +319,26 @@ BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, float rz)
         const mtx::Sample s = mtx::mx_sample(l.mx, wo, rx, ry);
         return { s.wi, s.weight, s.pdf };
     }
-    if (l.kind == kLobeDiffuse) {
+    if (l.kind == kLobePhong) {
+        // Phong::sample (shading.cpp:342-362)
+        const float cosNO = dot(l.N, wo);
+        if (cosNO > 0) {
+            const V3 R      = l.N * (2 * cosNO) - wo;
+            const float phi = 2 * float(M_PI) * rx;
+            float sp, cp;
+            ofast_sincosf(phi, &sp, &cp);
+            const float cosTheta  = ofast_safe_powf(ry, 1 / (l.alpha + 1));
+            const float sinTheta2 = 1 - cosTheta * cosTheta;
+            const float sinTheta  = sinTheta2 > 0 ? sqrtf(sinTheta2) : float(0.0f);
+            const V3 wi = frame_get(frame_from_normal(R), cp * sinTheta, sp * sinTheta, cosTheta);
+            const BsdfSample e = lobe_eval(l, wo, wi);
+            r.wi     = wi;
+            r.weight = e.weight;
+            r.pdf    = e.pdf;
+        }
+        return r;
+    }
+    if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent) {
         to_unit_disk(rx, ry);
         const float cos_theta = sqrtf(std::max<float>(1 - rx * rx - ry * ry, 0.0f));
         r.wi     = frame_get(frame_from_normal(l.N), rx, ry, cos_theta);
@@ -368,8 +402,8 @@ V3 lobe_albedo(const Lobe& l, V3 wo)
 {
     if (l.kind == kLobeMx)
         return mtx::mx_albedo(l.mx);
-    if (l.kind == kLobeDiffuse)
-        return V3(1.0f);
+    if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent || l.kind == kLobePhong)
+        return V3(1.0f);  // BSDF::get_albedo (src/testrender/shading.h:287-290)
     // Microfacet::get_albedo (shading.cpp:580-588)
     if (l.refract == 2)
         return V3(1.0f);

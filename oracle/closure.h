@@ -6,7 +6,11 @@
 
 namespace oref {
 
-enum LobeKind { kLobeDiffuse = 0, kLobeGGXRefl = 1, kLobeBeckmannRefl = 2, kLobeMx = 3 };
+enum LobeKind {
+    kLobeDiffuse = 0, kLobeGGXRefl = 1, kLobeBeckmannRefl = 2, kLobeMx = 3,
+    kLobeTranslucent = 4,  // Diffuse<1>: N holds the closure's N negated
+    kLobePhong = 5         // alpha holds the exponent
+};
 
 struct Lobe {
     int kind;

@@ -847,10 +847,16 @@ struct Exec {
             } else {
                 Lobe& l  = nd.lobe;
                 l.weight = V3(1.0f);
-                if (cname == "diffuse") {
-                    l.kind  = kLobeDiffuse;
-                    l.N     = T(L, a(2));
+                if (cname == "diffuse" || cname == "translucent") {
+                    // translucent is testrender's Diffuse<1>: the lobe about -N
+                    l.kind  = cname == "diffuse" ? kLobeDiffuse : kLobeTranslucent;
+                    l.N     = cname == "diffuse" ? T(L, a(2)) : -T(L, a(2));
                     l.alpha = 0.0f;
+                    l.eta   = 0.0f;
+                } else if (cname == "phong") {
+                    l.kind  = kLobePhong;
+                    l.N     = T(L, a(2));
+                    l.alpha = F(L, a(3));
                     l.eta   = 0.0f;
                 } else {
                     // microfacet(dist, N, U, xalpha, yalpha, eta, refract)
@@ -1271,9 +1277,13 @@ void oref_group_closures(void* h, const float* P, const float* N, const float* d
                 *p++ = a.y;
                 *p++ = a.z;
             };
-            if (l.kind == kLobeDiffuse) {
-                v[0] = 3;  // DIFFUSE_ID
+            if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent) {
+                v[0] = l.kind == kLobeDiffuse ? 3 : 5;  // DIFFUSE_ID, TRANSLUCENT_ID
+                put3(l.kind == kLobeDiffuse ? l.N : -l.N);  // the closure's N
+            } else if (l.kind == kLobePhong) {
+                v[0] = 6;  // PHONG_ID: N, exponent
                 put3(l.N);
+                *p++ = l.alpha;
             } else if (l.kind == kLobeGGXRefl || l.kind == kLobeBeckmannRefl) {
                 v[0] = 8;  // MICROFACET_ID: dist, N, U, xalpha, yalpha, eta, refract
                 *p++ = l.kind == kLobeBeckmannRefl ? 1.0f : 0.0f;

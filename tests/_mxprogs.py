@@ -174,6 +174,22 @@ def programs():
                         eta=1.0 / 1.33)
     out.append(("mx_glass", w.done(w.add(w.layer(w.sheen(w.lin(z, 0.1, 0.5)), glass),
                                          w.mul(w.fconst(0.5), inner)))))
+
+    # Beckmann lobes, sampled as well: an anisotropic one under a sheen and
+    # an isotropic one beside it
+    w = _W("mx_beckmann")
+    x, y, z = w.f
+    bk = w.ggx_aniso("dPdu", w.lin(x, 0.05, 0.5), w.lin(y, 0.05, 0.5), dist="beckmann")
+    iso = w.ggx_aniso(w.zero_v(), w.lin(z, 0.1, 0.4), w.lin(z, 0.1, 0.4), dist="beckmann")
+    out.append(("mx_beckmann", w.done(w.add(w.layer(w.sheen(w.lin(z, 0.1, 0.5)), bk),
+                                            w.mul(w.fconst(0.4), iso)))))
+
+    # Phong under a sheen, plus a translucent lobe (diffuse about -N)
+    w = _W("mx_phong")
+    x, y, z = w.f
+    ph = w.closure("phong", "N", w.lin(x, 2.0, 60.0))
+    out.append(("mx_phong", w.done(w.add(w.layer(w.sheen(w.lin(z, 0.1, 0.5)), ph),
+                                         w.mul(w.fconst(0.3), w.closure("translucent", "N"))))))
     return out
 
 
