@@ -42,7 +42,10 @@ namespace oslmi {
 enum ClosureId {
     kClosureDiffuse = 1, kClosureMicrofacetGGX = 2, kClosureMicrofacetBeckmann = 3,
     kClosureTranslucent = 4, kClosurePhong = 5,
-    kClosureMxDielectric = 6, kClosureMxSchlick = 7, kClosureMxSheen = 8
+    // delta lobes: one direction with an infinite pdf, nothing to evaluate
+    kClosureReflection = 6, kClosureFresnelReflection = 7, kClosureRefraction = 8,
+    kClosureTransparent = 9,
+    kClosureMxDielectric = 10, kClosureMxSchlick = 11, kClosureMxSheen = 12
 };
 
 struct Lobe {
@@ -323,6 +326,11 @@ OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
     }
     if (l.id == kClosurePhong)
         return phong_eval(l, wo, wi);
+    if (l.id >= kClosureReflection && l.id <= kClosureTransparent) {
+        r.weight = V3(0.0f);
+        r.pdf    = 0.0f;
+        return r;
+    }
     Frame tf       = lobe_frame(l);
     V3 wo_l        = tolocal(tf, wo);
     V3 wi_l        = tolocal(tf, wi);
@@ -391,8 +399,16 @@ OSLMI_DEV V3 lobe_albedo(const Lobe& l, V3 wo)
     if (l.id >= kClosureMxDielectric)
         return mtx::mx_albedo(l.mx);
 #endif
-    if (l.id == kClosureDiffuse || l.id == kClosureTranslucent || l.id == kClosurePhong)
+    if (l.id == kClosureDiffuse || l.id == kClosureTranslucent || l.id == kClosurePhong
+        || l.id == kClosureTransparent)
         return V3(1.0f);  // BSDF::get_albedo (src/testrender/shading.h:287-290)
+    if (l.id == kClosureReflection || l.id == kClosureFresnelReflection) {
+        // Reflection::get_albedo (shading.cpp:1086-1092)
+        const float cosNO = dot(l.N, wo);
+        return V3(cosNO > 0 ? fresnel_dielectric(cosNO, l.eta) : 1.0f);
+    }
+    if (l.id == kClosureRefraction)
+        return V3(1 - fresnel_dielectric(dot(l.N, wo), l.eta));  // shading.cpp:1117-1121
     // Microfacet::get_albedo (shading.cpp:580-588)
     const int rf = lobe_refract(l);
     if (rf == 2)
@@ -565,6 +581,30 @@ OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, fl
         return r;
     }
 #endif
+    if (l.id >= kClosureReflection && l.id <= kClosureTransparent) {
+        // Reflection / Refraction / Transparent::sample (shading.cpp:1098-1149):
+        // the one direction, an infinite pdf
+        const float inf = __uint_as_float(0x7f800000u);
+        if (l.id == kClosureTransparent) {
+            r.wi     = -wo;
+            r.weight = V3(1.0f);
+            r.pdf    = inf;
+        } else if (l.id == kClosureRefraction) {
+            V3 wi;
+            const float Ft = fresnel_refraction(-wo, l.N, l.eta, &wi);
+            r.wi     = wi;
+            r.weight = V3(Ft);
+            r.pdf    = inf;
+        } else {
+            const float cosNO = dot(l.N, wo);
+            if (cosNO > 0) {
+                r.wi     = l.N * (2 * cosNO) - wo;
+                r.weight = V3(fresnel_dielectric(cosNO, l.eta));
+                r.pdf    = inf;
+            }
+        }
+        return r;
+    }
     if (l.id == kClosurePhong) {
         // Phong::sample (shading.cpp:342-362): a cosine-power lobe about the
         // mirror direction, then the eval there
@@ -717,6 +757,10 @@ OSLMI_DEV float export_id(int id)
     return id == kClosureDiffuse ? 3.0f                        // DIFFUSE_ID
            : id == kClosureTranslucent ? 5.0f                 // TRANSLUCENT_ID
            : id == kClosurePhong ? 6.0f                       // PHONG_ID
+           : id == kClosureReflection ? 9.0f                  // REFLECTION_ID
+           : id == kClosureFresnelReflection ? 10.0f          // FRESNEL_REFLECTION_ID
+           : id == kClosureRefraction ? 11.0f                 // REFRACTION_ID
+           : id == kClosureTransparent ? 12.0f                // TRANSPARENT_ID
            : id == kClosureMicrofacetGGX || id == kClosureMicrofacetBeckmann ? 8.0f  // MICROFACET_ID
            : id == kClosureMxDielectric ? 17.0f               // MX_DIELECTRIC_ID
            : id == kClosureMxSchlick ? 19.0f                  // MX_GENERALIZED_SCHLICK_ID

@@ -1236,6 +1236,21 @@ struct Emitter {
             strval(L, a(1), cname);
             if (cname == "diffuse") {
                 line(R + " = closure_diffuse(" + ex(L, a(2), K_V3) + ");");
+            } else if (cname == "reflection" || cname == "refraction") {
+                // reflection(N) is REFLECTION_ID, reflection(N, eta)
+                // FRESNEL_REFLECTION_ID, refraction(N, eta) REFRACTION_ID
+                const bool with_eta = op.args.size() >= 4;
+                if (cname == "refraction" && !with_eta) {
+                    fail("refraction closure needs 2 arguments");
+                    return;
+                }
+                const std::string id = cname == "refraction" ? "kClosureRefraction"
+                                       : with_eta            ? "kClosureFresnelReflection"
+                                                             : "kClosureReflection";
+                line(R + " = closure_delta(" + id + ", " + ex(L, a(2), K_V3) + ", "
+                     + (with_eta ? ex(L, a(3), K_FLOAT) : std::string("0.0f")) + ");");
+            } else if (cname == "transparent") {
+                line(R + " = closure_delta(kClosureTransparent, V3(0.0f), 0.0f);");
             } else if (cname == "translucent") {
                 line(R + " = closure_translucent(" + ex(L, a(2), K_V3) + ");");
             } else if (cname == "phong") {

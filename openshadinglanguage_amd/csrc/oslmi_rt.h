@@ -268,6 +268,26 @@ OSLMI_DEV LobeList<1> closure_phong(V3 N, float exponent)
 #endif
     return c;
 }
+// reflection(N) (eta 0: a perfect mirror), reflection(N, eta),
+// refraction(N, eta) and transparent(): testrender's delta lobes
+OSLMI_DEV LobeList<1> closure_delta(int id, V3 N, float eta)
+{
+    LobeList<1> c;
+    Lobe l;
+    l.id = id; l.weight = V3(1.0f); l.N = N; l.alpha = 0.0f; l.eta = eta;
+#ifdef OSLMI_CLOSURE_EXPORT
+    {
+        // ReflectionParams / RefractionParams: N, eta (none for transparent)
+        const float v[4] = { N.x, N.y, N.z, eta };
+        set_xp(l, v, id == kClosureTransparent ? 0 : id == kClosureReflection ? 3 : 4);
+    }
+#endif
+    c.add(l);
+#ifdef OSLMI_MX_CLOSURES
+    mark_component(c);
+#endif
+    return c;
+}
 OSLMI_DEV LobeList<1> closure_microfacet_ggx(V3 N, float alpha, float eta)
 {
     LobeList<1> c;

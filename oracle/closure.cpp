@@ -17,6 +17,8 @@
 // the oracle's own sincos (omath.h) stands in (parity unpinned at ulp level).
 #include "closure.h"
 
+#include <limits>
+
 #include <algorithm>
 #include <cfloat>
 
@@ -119,6 +121,8 @@ BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
         const float pdf = std::max<float>(dot(l.N, wi), 0.0f) * float(M_1_PI);
         return { V3(1.0f), pdf };
     }
+    if (l.kind >= kLobeReflection && l.kind <= kLobeTransparent)
+        return { V3(0.0f), 0.0f };
     if (l.kind == kLobePhong) {
         // Phong::eval (src/testrender/shading.cpp:323-340)
         const float cosNI = dot(l.N, wi);
@@ -319,6 +323,21 @@ BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, float rz)
         const mtx::Sample s = mtx::mx_sample(l.mx, wo, rx, ry);
         return { s.wi, s.weight, s.pdf };
     }
+    if (l.kind >= kLobeReflection && l.kind <= kLobeTransparent) {
+        // Reflection / Refraction / Transparent::sample (shading.cpp:1098-1149)
+        const float inf = std::numeric_limits<float>::infinity();
+        if (l.kind == kLobeTransparent)
+            return { -wo, V3(1.0f), inf };
+        if (l.kind == kLobeRefraction) {
+            V3 wi;
+            const float Ft = fresnel_refraction(-wo, l.N, l.eta, wi);
+            return { wi, V3(Ft), inf };
+        }
+        const float cosNO = dot(l.N, wo);
+        if (cosNO > 0)
+            return { l.N * (2 * cosNO) - wo, V3(fresnel_dielectric(cosNO, l.eta)), inf };
+        return r;
+    }
     if (l.kind == kLobePhong) {
         // Phong::sample (shading.cpp:342-362)
         const float cosNO = dot(l.N, wo);
@@ -402,8 +421,15 @@ V3 lobe_albedo(const Lobe& l, V3 wo)
 {
     if (l.kind == kLobeMx)
         return mtx::mx_albedo(l.mx);
-    if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent || l.kind == kLobePhong)
+    if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent || l.kind == kLobePhong
+        || l.kind == kLobeTransparent)
         return V3(1.0f);  // BSDF::get_albedo (src/testrender/shading.h:287-290)
+    if (l.kind == kLobeReflection || l.kind == kLobeFresnelReflection) {
+        const float cosNO = dot(l.N, wo);  // Reflection::get_albedo (shading.cpp:1086-1092)
+        return V3(cosNO > 0 ? fresnel_dielectric(cosNO, l.eta) : float(1.0f));
+    }
+    if (l.kind == kLobeRefraction)
+        return V3(1 - fresnel_dielectric(dot(l.N, wo), l.eta));  // shading.cpp:1117-1121
     // Microfacet::get_albedo (shading.cpp:580-588)
     if (l.refract == 2)
         return V3(1.0f);

@@ -9,7 +9,9 @@ namespace oref {
 enum LobeKind {
     kLobeDiffuse = 0, kLobeGGXRefl = 1, kLobeBeckmannRefl = 2, kLobeMx = 3,
     kLobeTranslucent = 4,  // Diffuse<1>: N holds the closure's N negated
-    kLobePhong = 5         // alpha holds the exponent
+    kLobePhong = 5,        // alpha holds the exponent
+    // delta lobes (one direction, infinite pdf, nothing to evaluate)
+    kLobeReflection = 6, kLobeFresnelReflection = 7, kLobeRefraction = 8, kLobeTransparent = 9
 };
 
 struct Lobe {

@@ -853,6 +853,16 @@ struct Exec {
                     l.N     = cname == "diffuse" ? T(L, a(2)) : -T(L, a(2));
                     l.alpha = 0.0f;
                     l.eta   = 0.0f;
+                } else if (cname == "reflection" || cname == "refraction"
+                           || cname == "transparent") {
+                    const bool with_eta = op.args.size() >= 4;
+                    l.kind  = cname == "transparent" ? kLobeTransparent
+                              : cname == "refraction" ? kLobeRefraction
+                              : with_eta              ? kLobeFresnelReflection
+                                                      : kLobeReflection;
+                    l.N     = cname == "transparent" ? V3(0.0f) : T(L, a(2));
+                    l.alpha = 0.0f;
+                    l.eta   = with_eta && cname != "transparent" ? F(L, a(3)) : float(0.0f);
                 } else if (cname == "phong") {
                     l.kind  = kLobePhong;
                     l.N     = T(L, a(2));
@@ -1280,6 +1290,14 @@ void oref_group_closures(void* h, const float* P, const float* N, const float* d
             if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent) {
                 v[0] = l.kind == kLobeDiffuse ? 3 : 5;  // DIFFUSE_ID, TRANSLUCENT_ID
                 put3(l.kind == kLobeDiffuse ? l.N : -l.N);  // the closure's N
+            } else if (l.kind >= kLobeReflection && l.kind <= kLobeTransparent) {
+                // REFLECTION_ID (N), FRESNEL_REFLECTION_ID and REFRACTION_ID
+                // (N, eta), TRANSPARENT_ID (none)
+                v[0] = 9 + (l.kind - kLobeReflection);
+                if (l.kind != kLobeTransparent)
+                    put3(l.N);
+                if (l.kind == kLobeFresnelReflection || l.kind == kLobeRefraction)
+                    *p++ = l.eta;
             } else if (l.kind == kLobePhong) {
                 v[0] = 6;  // PHONG_ID: N, exponent
                 put3(l.N);

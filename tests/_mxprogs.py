@@ -190,6 +190,16 @@ def programs():
     ph = w.closure("phong", "N", w.lin(x, 2.0, 60.0))
     out.append(("mx_phong", w.done(w.add(w.layer(w.sheen(w.lin(z, 0.1, 0.5)), ph),
                                          w.mul(w.fconst(0.3), w.closure("translucent", "N"))))))
+
+    # delta lobes beside a diffuse one: a Fresnel mirror under a sheen, a
+    # perfect mirror, a refraction and a transparent lobe
+    w = _W("mx_delta")
+    x, y, z = w.f
+    fres = w.closure("reflection", "N", w.lin(x, 1.2, 0.6))
+    out.append(("mx_delta", w.done(w.add(
+        w.add(w.layer(w.sheen(w.lin(z, 0.1, 0.5)), fres), w.mul(w.fconst(0.2), w.closure("reflection", "N"))),
+        w.add(w.mul(w.fconst(0.3), w.closure("refraction", "N", w.lin(y, 1.1, 0.5))),
+              w.add(w.mul(w.fconst(0.1), w.closure("transparent")), w.mul(w.fconst(0.4), w.diffuse())))))))
     return out
 
 

@@ -67,3 +67,42 @@ def test_translucent_is_diffuse_about_minus_n():
     tr = O.composite_eval(np.array([[4, 1, 1, 1, *(-N), 0.0, 0.0]]), wo, wi)
     np.testing.assert_allclose(tr[3], np.maximum(-(N @ wi), 0) / np.pi, rtol=1e-5, atol=1e-7)
     assert np.all(tr[0][tr[3] > 0] == 1.0)
+
+
+def test_delta_lobes():
+    """reflection / refraction / transparent (shading.cpp:1081-1149): the
+    mirror direction with the dielectric Fresnel weight (a perfect mirror
+    without eta), the Snell direction with the transmitted share, -wo; all
+    with an infinite pdf and nothing to evaluate. Reflected and refracted
+    weights of one interface sum to 1."""
+    rng = np.random.default_rng(4)
+    n = 2000
+    N = np.array([0.0, 0.0, 1.0])
+    wo = rng.normal(size=(3, n))
+    wo[2] = np.abs(wo[2]) + 0.05
+    wo /= np.linalg.norm(wo, axis=0)
+    rnd = rng.uniform(size=(3, n))
+    eta = 1.5
+    refl = O.composite_sample(np.array([[7, 1, 1, 1, *N, 0.0, eta]]), wo, rnd)
+    mirror = O.composite_sample(np.array([[6, 1, 1, 1, *N, 0.0, 0.0]]), wo, rnd)
+    refr = O.composite_sample(np.array([[8, 1, 1, 1, *N, 0.0, eta]]), wo, rnd)
+    tr = O.composite_sample(np.array([[9, 1, 1, 1, 0, 0, 0, 0.0, 0.0]]), wo, rnd)
+    want_r = 2 * wo[2] * N[:, None] - wo
+    np.testing.assert_allclose(refl[:3], want_r, atol=2e-7)
+    c = wo[2].astype(np.float64)
+    g = np.sqrt(eta * eta - 1 + c * c)
+    A, B = (g - c) / (g + c), (c * (g + c) - 1) / (c * (g - c) + 1)
+    F = 0.5 * A * A * (1 + B * B)
+    np.testing.assert_allclose(refl[3], F, rtol=2e-6)
+    np.testing.assert_allclose(mirror[3], 1.0)
+    np.testing.assert_allclose(refl[3] + refr[3], 1.0, rtol=2e-6)
+    # Snell: sin(theta_t) = sin(theta_i) / eta, below the surface, in the plane
+    sin_i = np.sqrt(1 - c * c)
+    sin_t = np.sqrt(1 - refr[2].astype(np.float64) ** 2)
+    np.testing.assert_allclose(sin_t, sin_i / eta, atol=2e-6)
+    assert np.all(refr[2] < 0)
+    np.testing.assert_allclose(tr[:3], -wo, atol=0)
+    for s in (refl, mirror, refr, tr):
+        assert np.all(np.isinf(s[6]))
+    ev = O.composite_eval(np.array([[7, 1, 1, 1, *N, 0.0, eta]]), wo, want_r)
+    assert np.all(ev == 0)
