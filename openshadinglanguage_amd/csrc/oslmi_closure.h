@@ -45,7 +45,11 @@ enum ClosureId {
     // delta lobes: one direction with an infinite pdf, nothing to evaluate
     kClosureReflection = 6, kClosureFresnelReflection = 7, kClosureRefraction = 8,
     kClosureTransparent = 9,
-    kClosureMxDielectric = 10, kClosureMxSchlick = 11, kClosureMxSheen = 12
+    // emission(): light, not a BSDF: kept in the list for the closure export
+    // (its weight is the emitted radiance) and ignored by prepare / eval /
+    // sample, as testrender adds it to Le instead of the CompositeBSDF
+    kClosureEmission = 10,
+    kClosureMxDielectric = 11, kClosureMxSchlick = 12, kClosureMxSheen = 13
 };
 
 struct Lobe {
@@ -326,7 +330,7 @@ OSLMI_DEV BsdfEval lobe_eval(const Lobe& l, V3 wo, V3 wi)
     }
     if (l.id == kClosurePhong)
         return phong_eval(l, wo, wi);
-    if (l.id >= kClosureReflection && l.id <= kClosureTransparent) {
+    if (l.id >= kClosureReflection && l.id <= kClosureEmission) {
         r.weight = V3(0.0f);
         r.pdf    = 0.0f;
         return r;
@@ -402,6 +406,8 @@ OSLMI_DEV V3 lobe_albedo(const Lobe& l, V3 wo)
     if (l.id == kClosureDiffuse || l.id == kClosureTranslucent || l.id == kClosurePhong
         || l.id == kClosureTransparent)
         return V3(1.0f);  // BSDF::get_albedo (src/testrender/shading.h:287-290)
+    if (l.id == kClosureEmission)
+        return V3(0.0f);  // never picked by the sampler
     if (l.id == kClosureReflection || l.id == kClosureFresnelReflection) {
         // Reflection::get_albedo (shading.cpp:1086-1092)
         const float cosNO = dot(l.N, wo);
@@ -581,6 +587,8 @@ OSLMI_DEV BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, fl
         return r;
     }
 #endif
+    if (l.id == kClosureEmission)
+        return r;
     if (l.id >= kClosureReflection && l.id <= kClosureTransparent) {
         // Reflection / Refraction / Transparent::sample (shading.cpp:1098-1149):
         // the one direction, an infinite pdf
@@ -761,6 +769,7 @@ OSLMI_DEV float export_id(int id)
            : id == kClosureFresnelReflection ? 10.0f          // FRESNEL_REFLECTION_ID
            : id == kClosureRefraction ? 11.0f                 // REFRACTION_ID
            : id == kClosureTransparent ? 12.0f                // TRANSPARENT_ID
+           : id == kClosureEmission ? 1.0f                    // EMISSION_ID
            : id == kClosureMicrofacetGGX || id == kClosureMicrofacetBeckmann ? 8.0f  // MICROFACET_ID
            : id == kClosureMxDielectric ? 17.0f               // MX_DIELECTRIC_ID
            : id == kClosureMxSchlick ? 19.0f                  // MX_GENERALIZED_SCHLICK_ID

@@ -1251,6 +1251,8 @@ struct Emitter {
                      + (with_eta ? ex(L, a(3), K_FLOAT) : std::string("0.0f")) + ");");
             } else if (cname == "transparent") {
                 line(R + " = closure_delta(kClosureTransparent, V3(0.0f), 0.0f);");
+            } else if (cname == "emission") {
+                line(R + " = closure_delta(kClosureEmission, V3(0.0f), 0.0f);");
             } else if (cname == "translucent") {
                 line(R + " = closure_translucent(" + ex(L, a(2), K_V3) + ");");
             } else if (cname == "phong") {

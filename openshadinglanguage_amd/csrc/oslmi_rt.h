@@ -269,7 +269,8 @@ OSLMI_DEV LobeList<1> closure_phong(V3 N, float exponent)
     return c;
 }
 // reflection(N) (eta 0: a perfect mirror), reflection(N, eta),
-// refraction(N, eta) and transparent(): testrender's delta lobes
+// refraction(N, eta) and transparent(): testrender's delta lobes; also
+// emission() (no parameters)
 OSLMI_DEV LobeList<1> closure_delta(int id, V3 N, float eta)
 {
     LobeList<1> c;
@@ -279,7 +280,9 @@ OSLMI_DEV LobeList<1> closure_delta(int id, V3 N, float eta)
     {
         // ReflectionParams / RefractionParams: N, eta (none for transparent)
         const float v[4] = { N.x, N.y, N.z, eta };
-        set_xp(l, v, id == kClosureTransparent ? 0 : id == kClosureReflection ? 3 : 4);
+        set_xp(l, v, (id == kClosureTransparent || id == kClosureEmission) ? 0
+                     : id == kClosureReflection                              ? 3
+                                                                             : 4);
     }
 #endif
     c.add(l);

@@ -121,7 +121,7 @@ BsdfSample lobe_eval(const Lobe& l, V3 wo, V3 wi)
         const float pdf = std::max<float>(dot(l.N, wi), 0.0f) * float(M_1_PI);
         return { V3(1.0f), pdf };
     }
-    if (l.kind >= kLobeReflection && l.kind <= kLobeTransparent)
+    if (l.kind >= kLobeReflection && l.kind <= kLobeEmission)
         return { V3(0.0f), 0.0f };
     if (l.kind == kLobePhong) {
         // Phong::eval (src/testrender/shading.cpp:323-340)
@@ -323,6 +323,8 @@ BsdfDirSample lobe_sample(const Lobe& l, V3 wo, float rx, float ry, float rz)
         const mtx::Sample s = mtx::mx_sample(l.mx, wo, rx, ry);
         return { s.wi, s.weight, s.pdf };
     }
+    if (l.kind == kLobeEmission)
+        return r;
     if (l.kind >= kLobeReflection && l.kind <= kLobeTransparent) {
         // Reflection / Refraction / Transparent::sample (shading.cpp:1098-1149)
         const float inf = std::numeric_limits<float>::infinity();
@@ -424,6 +426,8 @@ V3 lobe_albedo(const Lobe& l, V3 wo)
     if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent || l.kind == kLobePhong
         || l.kind == kLobeTransparent)
         return V3(1.0f);  // BSDF::get_albedo (src/testrender/shading.h:287-290)
+    if (l.kind == kLobeEmission)
+        return V3(0.0f);  // never picked by the sampler
     if (l.kind == kLobeReflection || l.kind == kLobeFresnelReflection) {
         const float cosNO = dot(l.N, wo);  // Reflection::get_albedo (shading.cpp:1086-1092)
         return V3(cosNO > 0 ? fresnel_dielectric(cosNO, l.eta) : float(1.0f));

@@ -11,7 +11,10 @@ enum LobeKind {
     kLobeTranslucent = 4,  // Diffuse<1>: N holds the closure's N negated
     kLobePhong = 5,        // alpha holds the exponent
     // delta lobes (one direction, infinite pdf, nothing to evaluate)
-    kLobeReflection = 6, kLobeFresnelReflection = 7, kLobeRefraction = 8, kLobeTransparent = 9
+    kLobeReflection = 6, kLobeFresnelReflection = 7, kLobeRefraction = 8, kLobeTransparent = 9,
+    // emission(): light, not a BSDF; kept for the closure export (weight =
+    // emitted radiance), ignored by prepare / eval / sample
+    kLobeEmission = 10
 };
 
 struct Lobe {

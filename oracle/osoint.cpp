@@ -853,6 +853,11 @@ struct Exec {
                     l.N     = cname == "diffuse" ? T(L, a(2)) : -T(L, a(2));
                     l.alpha = 0.0f;
                     l.eta   = 0.0f;
+                } else if (cname == "emission") {
+                    l.kind  = kLobeEmission;
+                    l.N     = V3(0.0f);
+                    l.alpha = 0.0f;
+                    l.eta   = 0.0f;
                 } else if (cname == "reflection" || cname == "refraction"
                            || cname == "transparent") {
                     const bool with_eta = op.args.size() >= 4;
@@ -1290,6 +1295,8 @@ void oref_group_closures(void* h, const float* P, const float* N, const float* d
             if (l.kind == kLobeDiffuse || l.kind == kLobeTranslucent) {
                 v[0] = l.kind == kLobeDiffuse ? 3 : 5;  // DIFFUSE_ID, TRANSLUCENT_ID
                 put3(l.kind == kLobeDiffuse ? l.N : -l.N);  // the closure's N
+            } else if (l.kind == kLobeEmission) {
+                v[0] = 1;  // EMISSION_ID: no parameters
             } else if (l.kind >= kLobeReflection && l.kind <= kLobeTransparent) {
                 // REFLECTION_ID (N), FRESNEL_REFLECTION_ID and REFRACTION_ID
                 // (N, eta), TRANSPARENT_ID (none)

@@ -200,6 +200,13 @@ def programs():
         w.add(w.layer(w.sheen(w.lin(z, 0.1, 0.5)), fres), w.mul(w.fconst(0.2), w.closure("reflection", "N"))),
         w.add(w.mul(w.fconst(0.3), w.closure("refraction", "N", w.lin(y, 1.1, 0.5))),
               w.add(w.mul(w.fconst(0.1), w.closure("transparent")), w.mul(w.fconst(0.4), w.diffuse())))))))
+
+    # emission beside a BSDF: exported with its weight, ignored by eval and
+    # sampling
+    w = _W("mx_emission")
+    x, y, z = w.f
+    out.append(("mx_emission", w.done(w.add(w.mul(w.color("dPdu"), w.closure("emission")),
+                                            w.layer(w.sheen(w.lin(z, 0.1, 0.5)), w.diffuse())))))
     return out
 
 
